@@ -1,0 +1,124 @@
+/*
+ * ptx.h -- C ABI of the MI355X-native ptychography forward/adjoint library
+ * (libptychox_hip.so, HIP for gfx950).
+ *
+ * It replaces the pybind11 modules ``ptychoX.{base,fpm,coded}._C`` that the
+ * reference's operators call (python/ptychoX/csrc/{base,fpm,coded}_extension.cpp) with plain
+ * C entry points: device pointers, sizes and a stream, no torch types.
+ *
+ * Conventions
+ *  - complex64 buffers are interleaved (re, im) float pairs, row-major,
+ *    contiguous, on the device;  float buffers likewise.
+ *  - positions / LED indices are device int64 [batch][2] = (x, y), zero-based,
+ *    x the column (python/ptychoX/base/ops.py:61-62).
+ *  - every call is stream-ordered on ``stream`` (a hipStream_t; NULL = the
+ *    default stream), does not synchronise and allocates no device memory:
+ *    scratch space is the caller's ``workspace`` of at least the size the
+ *    matching *_workspace() query returns.
+ *  - transform lengths (probe/pupil/frame sides) are powers of two in
+ *    [PTX_MIN_N, PTX_MAX_N]; other sizes return PTX_ERR_UNSUPPORTED.
+ *  - a non-zero status leaves a message for ptx_last_error() (thread-local).
+ *  - outputs are overwritten unless PTX_FLAG_ACCUMULATE is given.
+ *  - calls are thread-safe: the library holds no mutable global state besides
+ *    the thread-local error string.
+ */
+#ifndef PTX_H_
+#define PTX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTX_ABI_VERSION 1
+#define PTX_MIN_N 8
+#define PTX_MAX_N 2048
+
+typedef enum ptx_status {
+  PTX_OK = 0,
+  PTX_ERR_INVALID_ARGUMENT = 1,
+  PTX_ERR_UNSUPPORTED = 2,
+  PTX_ERR_OUT_OF_RANGE = 3,
+  PTX_ERR_WORKSPACE = 4,
+  PTX_ERR_HIP = 5
+} ptx_status;
+
+/* Object / spectrum gradient by a sorted-segment gather in ascending view
+ * order instead of float atomics: bitwise reproducible, same summation order
+ * as the reference torch loop (python/ptychoX/base/ops.py:100-105). */
+#define PTX_FLAG_DETERMINISTIC 1u
+/* Add into the gradient outputs instead of overwriting them. */
+#define PTX_FLAG_ACCUMULATE 2u
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* ptx_last_error(void);
+/* Library version string and ABI version (PTX_ABI_VERSION). */
+const char* ptx_version(void);
+int ptx_abi_version(void);
+
+/* ------------------------------------------------------------------ FFT
+ * Batched in-place 2-D C2C transform of ``batch`` frames of h x w,
+ * unnormalised in both directions (inverse != 0 uses exp(+i)).
+ * Replaces the cuFFT plan cache + execution of
+ * python/ptychoX/csrc/base_kernel.cu:43-83,225-235 (and the fpm/coded copies).
+ * Plans are compile-time, so there is nothing to cache or clear. */
+ptx_status ptx_fft2d(void* data, int64_t batch, int64_t h, int64_t w, int inverse, void* stream);
+
+/* ------------------------------------------------------------------ conventional
+ * Forward of ptychoX_base: far_field[b] = fftshift(fft2(sample[y_b:y_b+h, x_b:x_b+w] * probe)),
+ * amplitude[b] = |far_field[b]|.
+ * Replaces ptychox_base_forward_cuda (python/ptychoX/csrc/base_kernel.cu:302-329,
+ * bound at base_extension.cpp:21-32) and the torch branch base/ops.py:60-66.
+ * The reference also returns the patch stack; this library re-gathers patches
+ * from ``sample`` in the backward instead of storing them.
+ * far_field: [batch, h, w] complex64 out; amplitude: [batch, h, w] float out. */
+ptx_status ptx_base_forward(const void* sample, int64_t sample_h, int64_t sample_w,
+                            const void* probe, int64_t probe_h, int64_t probe_w,
+                            const int64_t* positions_xy, int64_t batch,
+                            void* far_field, float* amplitude, void* stream);
+
+/* Workspace bytes ptx_base_backward needs. */
+size_t ptx_base_backward_workspace(int64_t sample_h, int64_t sample_w, int64_t probe_h, int64_t probe_w,
+                                   int64_t batch, unsigned flags);
+
+/* Backward of ptychoX_base: with Gamma = g * Phi / max(|Phi|, FLT_EPSILON) and
+ * chi_b = N * ifft2(ifftshift(Gamma_b)):
+ *   grad_sample[y_b:y_b+h, x_b:x_b+w] += chi_b * conj(probe),
+ *   grad_probe += sum_b chi_b * conj(patch_b).
+ * Replaces ptychox_base_backward_cuda (python/ptychoX/csrc/base_kernel.cu:331-357,
+ * bound at base_extension.cpp:34-60) and the torch branch base/ops.py:93-105.
+ * grad_output: [batch, h, w] float; far_field: forward's output;
+ * grad_sample: [sample_h, sample_w] complex64; grad_probe: [h, w] complex64. */
+ptx_status ptx_base_backward(const float* grad_output, const void* far_field,
+                             const void* sample, int64_t sample_h, int64_t sample_w,
+                             const void* probe, int64_t probe_h, int64_t probe_w,
+                             const int64_t* positions_xy, int64_t batch,
+                             void* grad_sample, void* grad_probe,
+                             void* workspace, size_t workspace_bytes, unsigned flags, void* stream);
+
+/* Workspace bytes ptx_base_step needs. */
+size_t ptx_base_step_workspace(int64_t sample_h, int64_t sample_w, int64_t probe_h, int64_t probe_w,
+                               int64_t batch, unsigned flags);
+
+/* Fused training step of the conventional operator with the amplitude loss of
+ * the reference driver (python/train_ptychography.py:549-558: ptychoX_base ->
+ * F.mse_loss -> backward), in three passes over the batch:
+ *   L = loss_scale * sum_{b,i} (A_bi - Y_bi)^2   (loss_scale = 1/(batch*h*w) is mse_loss "mean"),
+ *   loss_per_position[b] = mean_i (A_bi - Y_bi)^2,
+ *   grad_sample, grad_probe = dL/d(sample, probe) in the PyTorch convention.
+ * measured: [batch, h, w] float;  amplitude: optional [batch, h, w] float out (NULL to skip). */
+ptx_status ptx_base_step(const void* sample, int64_t sample_h, int64_t sample_w,
+                         const void* probe, int64_t probe_h, int64_t probe_w,
+                         const int64_t* positions_xy, int64_t batch,
+                         const float* measured, float loss_scale,
+                         float* loss_per_position, float* amplitude,
+                         void* grad_sample, void* grad_probe,
+                         void* workspace, size_t workspace_bytes, unsigned flags, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTX_H_ */

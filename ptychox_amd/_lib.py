@@ -1,0 +1,115 @@
+"""ctypes binding of the C ABI in ``include/ptx.h`` (libptychox_hip.so).
+
+The library is loaded after ``torch`` so that it binds to the HIP runtime torch
+already mapped (same ``libamdhip64.so.7`` soname): one runtime, one set of
+streams.  A missing or unloadable library raises -- there is no fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import torch  # noqa: F401  (loads the HIP runtime first)
+
+LIB_PATH = Path(__file__).resolve().parent / "libptychox_hip.so"
+
+PTX_FLAG_DETERMINISTIC = 1
+PTX_FLAG_ACCUMULATE = 2
+
+_STATUS = {
+    0: "PTX_OK",
+    1: "PTX_ERR_INVALID_ARGUMENT",
+    2: "PTX_ERR_UNSUPPORTED",
+    3: "PTX_ERR_OUT_OF_RANGE",
+    4: "PTX_ERR_WORKSPACE",
+    5: "PTX_ERR_HIP",
+}
+
+_p = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_sz = ctypes.c_size_t
+_u = ctypes.c_uint
+_f = ctypes.c_float
+_i = ctypes.c_int
+_st = ctypes.c_int
+
+# name -> (restype, argtypes); mirrors include/ptx.h
+SIGNATURES: dict[str, tuple] = {
+    "ptx_last_error": (ctypes.c_char_p, []),
+    "ptx_version": (ctypes.c_char_p, []),
+    "ptx_abi_version": (_i, []),
+    "ptx_fft2d": (_st, [_p, _i64, _i64, _i64, _i, _p]),
+    "ptx_base_forward": (_st, [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _p, _p]),
+    "ptx_base_backward_workspace": (_sz, [_i64, _i64, _i64, _i64, _i64, _u]),
+    "ptx_base_backward": (
+        _st,
+        [_p, _p, _p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _p, _p, _sz, _u, _p],
+    ),
+    "ptx_base_step_workspace": (_sz, [_i64, _i64, _i64, _i64, _i64, _u]),
+    "ptx_base_step": (
+        _st,
+        [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _f, _p, _p, _p, _p, _p, _sz, _u, _p],
+    ),
+}
+
+_lock = threading.Lock()
+_lib: ctypes.CDLL | None = None
+
+
+class NativeLibraryError(RuntimeError):
+    """The HIP library is missing, failed to load, or a call returned an error."""
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise NativeLibraryError(
+                    f"{LIB_PATH.name} is not built; run `python -m ptychox_amd.build` (hipcc, gfx950)"
+                )
+            try:
+                handle = ctypes.CDLL(str(LIB_PATH))
+            except OSError as exc:  # pragma: no cover - depends on the runtime
+                raise NativeLibraryError(f"cannot load {LIB_PATH}: {exc}") from exc
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(handle, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = handle
+    return _lib
+
+
+def last_error() -> str:
+    return lib().ptx_last_error().decode(errors="replace")
+
+
+def call(name: str, *args) -> None:
+    """Call a status-returning entry point; raise with the library's message on error."""
+    st = getattr(lib(), name)(*args)
+    if st != 0:
+        msg = last_error()
+        code = _STATUS.get(st, str(st))
+        if st in (1, 3):
+            raise ValueError(f"{name}: {msg} ({code})")
+        raise NativeLibraryError(f"{name}: {msg} ({code})")
+
+
+def query(name: str, *args) -> int:
+    return int(getattr(lib(), name)(*args))
+
+
+def ptr(t: torch.Tensor | None) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def stream_of(device: torch.device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def workspace(nbytes: int, device: torch.device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)

@@ -1,0 +1,95 @@
+"""Input validation and small helpers shared by the operators.
+
+Messages and exception types follow the reference
+(``python/ptychoX/_utils.py:24-46,99-101``) so callers see the same errors.
+"""
+
+from __future__ import annotations
+
+import os
+
+import torch
+
+PTX_MIN_N = 8
+PTX_MAX_N = 2048
+
+_DETERMINISTIC = os.environ.get("PTYCHOX_DETERMINISTIC", "0") not in ("", "0", "false", "False")
+
+
+def check_complex_scan_inputs(
+    first: torch.Tensor,
+    second: torch.Tensor,
+    positions_xy: torch.Tensor,
+    first_name: str = "sample",
+    second_name: str = "probe",
+) -> None:
+    """Same checks and messages as ``python/ptychoX/_utils.py:24-42``."""
+    if first.ndim != 2 or second.ndim != 2:
+        raise ValueError(f"{first_name} and {second_name} must be 2D complex tensors.")
+    if not first.is_complex() or not second.is_complex():
+        raise TypeError(f"{first_name} and {second_name} must be complex tensors.")
+    if first.device != second.device:
+        raise ValueError(f"{first_name} and {second_name} must live on the same device.")
+    if first.dtype != second.dtype:
+        raise ValueError(f"{first_name} and {second_name} must have the same dtype.")
+    if positions_xy.ndim != 2 or positions_xy.shape[-1] != 2:
+        raise ValueError("positions_xy must have shape [batch, 2].")
+    if positions_xy.device.type != "cpu" and positions_xy.device != first.device:
+        raise ValueError(f"positions_xy must be on CPU or on the same device as {first_name}.")
+
+
+def can_use_native(first: torch.Tensor, second: torch.Tensor, requested: bool) -> bool:
+    """``use_cuda_kernels=True`` on complex64 GPU tensors selects the HIP path
+    (the rule of ``python/ptychoX/_utils.py:45-46``; the flag keeps its name
+    for drop-in compatibility and means "native kernels" on ROCm)."""
+    return bool(requested) and first.is_cuda and first.dtype == torch.complex64 and second.dtype == torch.complex64
+
+
+def native_size_ok(n: int) -> bool:
+    return PTX_MIN_N <= n <= PTX_MAX_N and (n & (n - 1)) == 0
+
+
+def require_native_frame(h: int, w: int, what: str) -> None:
+    if not (native_size_ok(h) and native_size_ok(w)):
+        raise ValueError(
+            f"native HIP path: {what} {h}x{w} unsupported; sides must be powers of two in "
+            f"[{PTX_MIN_N}, {PTX_MAX_N}] (call with use_cuda_kernels=False for other sizes)"
+        )
+
+
+def check_windows(positions_xy: torch.Tensor, h: int, w: int, H: int, W: int) -> None:
+    """Every scan window ``[y, y+h) x [x, x+w)`` must lie inside the ``H x W`` sample.
+
+    The reference has no such check (its torch slice shrinks and ``torch.stack``
+    fails; its CUDA kernel reads out of bounds, python/ptychoX/csrc/base_kernel.cu:108-137).
+    One 4-number device-to-host copy when the positions live on the GPU.
+    """
+    if positions_xy.numel() == 0:
+        return
+    ext = torch.stack(
+        [positions_xy[:, 0].min(), positions_xy[:, 1].min(), positions_xy[:, 0].max(), positions_xy[:, 1].max()]
+    ).tolist()
+    xmin, ymin, xmax, ymax = ext
+    if xmin < 0 or ymin < 0 or xmax + w > W or ymax + h > H:
+        raise ValueError(
+            f"scan window out of range: positions x in [{xmin}, {xmax}], y in [{ymin}, {ymax}] with a "
+            f"{h}x{w} probe do not fit a {H}x{W} sample."
+        )
+
+
+def deterministic() -> bool:
+    """Sorted-segment (atomic-free) object accumulation: on when
+    ``set_deterministic(True)``, ``PTYCHOX_DETERMINISTIC=1`` or
+    ``torch.use_deterministic_algorithms(True)``."""
+    return _DETERMINISTIC or torch.are_deterministic_algorithms_enabled()
+
+
+def set_deterministic(enabled: bool) -> None:
+    global _DETERMINISTIC
+    _DETERMINISTIC = bool(enabled)
+
+
+def matlab_positions_to_torch(positions_xy_1based: torch.Tensor) -> torch.Tensor:
+    """Convert MATLAB one-based ``(x, y)`` scan positions to zero-based int64
+    (``python/ptychoX/_utils.py:99-101``)."""
+    return positions_xy_1based.to(torch.long) - 1

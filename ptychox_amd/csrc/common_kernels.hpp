@@ -1,0 +1,163 @@
+// Small helper kernels shared by the operators: fills, fixed-order partial
+// reductions, block reductions, and the sorted-segment (deterministic)
+// scatter-add of per-view windows into the object.
+#pragma once
+
+#include "fft_core.hpp"
+
+namespace ptx {
+
+constexpr float kFltEps = 1.1920928955078125e-07f;  // FLT_EPSILON, python/ptychoX/base/ops.py:93
+
+__device__ __forceinline__ bool window_ok(int64_t x0, int64_t y0, int h, int w, int64_t H, int64_t W) {
+  return x0 >= 0 && y0 >= 0 && x0 + w <= W && y0 + h <= H;
+}
+
+// graph-safe zero fill (no hipMemsetAsync node)
+static __global__ void __launch_bounds__(256) k_fill_zero2(float2* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = c_zero();
+}
+inline void fill_zero2(float2* p, int64_t n, hipStream_t s) {
+  if (n > 0) k_fill_zero2<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(p, n);
+}
+
+// out[i] (+)= sum_{c < nparts} part[c*n + i], summed in ascending c (deterministic)
+static __global__ void __launch_bounds__(256) k_sum_parts2(const float2* __restrict__ part, int64_t nparts, int64_t n,
+                                                    float2* __restrict__ out, int accumulate) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float2 s = accumulate ? out[i] : c_zero();
+  for (int64_t c = 0; c < nparts; ++c) s = c_add(s, part[c * n + i]);
+  out[i] = s;
+}
+inline void sum_parts2(const float2* part, int64_t nparts, int64_t n, float2* out, int accumulate, hipStream_t s) {
+  if (n > 0) k_sum_parts2<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(part, nparts, n, out, accumulate);
+}
+
+// out[b] = scale * sum_{k < nparts} part[b*nparts + k], in ascending k
+static __global__ void __launch_bounds__(256) k_loss_rows(const float* __restrict__ part, int64_t nparts, int64_t batch,
+                                                   float scale, float* __restrict__ out) {
+  const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (b >= batch) return;
+  float s = 0.0f;
+  for (int64_t k = 0; k < nparts; ++k) s += part[b * nparts + k];
+  out[b] = s * scale;
+}
+inline void loss_rows(const float* part, int64_t nparts, int64_t batch, float scale, float* out, hipStream_t s) {
+  if (batch > 0) k_loss_rows<<<dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, s>>>(part, nparts, batch, scale, out);
+}
+
+// Sum of x over the work-group (blockDim.x a power of two <= 1024); result valid in
+// every thread.  Shuffle offsets stay below blockDim.x so that work-groups smaller
+// than a wave never read inactive lanes.
+__device__ __forceinline__ float block_sum(float x, float* red /* >= 16 floats of LDS */) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    if (off < (int)blockDim.x) x += __shfl_xor(x, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nwaves = (blockDim.x + 63) >> 6;
+  if (lane == 0) red[wave] = x;
+  __syncthreads();
+  float s = 0.0f;
+  for (int w = 0; w < nwaves; ++w) s += red[w];  // fixed order
+  __syncthreads();
+  return s;
+}
+
+// ---------------------------------------------------------------- sorted-segment scatter
+
+constexpr int kTileH = 16;   // object rows per gather tile
+constexpr int kTileW = 128;  // object columns per gather tile
+
+// For every object tile, the views whose h x w window overlaps it, in ascending
+// view order.  lists: [ntiles, batch] int32; counts: [ntiles].
+static __global__ void __launch_bounds__(256) k_tile_lists(const int64_t* __restrict__ pos, int64_t batch, int h, int w,
+                                                    int64_t tiles_x, int* __restrict__ lists,
+                                                    int* __restrict__ counts) {
+  __shared__ int wave_tot[4];
+  const int64_t tile = blockIdx.x;
+  const int64_t ty0 = (tile / tiles_x) * kTileH, tx0 = (tile % tiles_x) * kTileW;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int count = 0;
+  for (int64_t base = 0; base < batch; base += 256) {
+    const int64_t b = base + threadIdx.x;
+    bool hit = false;
+    if (b < batch) {
+      const int64_t x0 = pos[2 * b], y0 = pos[2 * b + 1];
+      hit = y0 < ty0 + kTileH && y0 + h > ty0 && x0 < tx0 + kTileW && x0 + w > tx0;
+    }
+    const unsigned long long m = __ballot(hit);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_tot[wave] = __popcll(m);
+    __syncthreads();
+    int off = count;
+    for (int k = 0; k < wave; ++k) off += wave_tot[k];
+    if (hit) lists[tile * batch + off + before] = (int)b;
+    count += wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[tile] = count;
+}
+
+// grad[Y, X] (+)= sum over listed views b (ascending) of contrib[b, Y - y_b, X - x_b]
+static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __restrict__ contrib, int h, int w,
+                                                     const int64_t* __restrict__ pos, int64_t batch,
+                                                     const int* __restrict__ lists, const int* __restrict__ counts,
+                                                     int64_t tiles_x, float2* __restrict__ grad, int64_t H, int64_t W,
+                                                     int accumulate) {
+  constexpr int RPT = kTileH * kTileW / 256;  // rows per thread
+  const int64_t tile = blockIdx.x;
+  const int64_t ty0 = (tile / tiles_x) * kTileH, tx0 = (tile % tiles_x) * kTileW;
+  const int64_t X = tx0 + (threadIdx.x % kTileW);
+  const int64_t Y0 = ty0 + threadIdx.x / kTileW;
+  float2 acc[RPT];
+#pragma unroll
+  for (int j = 0; j < RPT; ++j) acc[j] = c_zero();
+  const int n = counts[tile];
+  for (int k = 0; k < n; ++k) {
+    const int64_t b = lists[tile * batch + k];
+    const int64_t x0 = pos[2 * b], y0 = pos[2 * b + 1];
+    const int64_t cx = X - x0;
+    if (cx < 0 || cx >= w) continue;
+    const float2* src = contrib + (b * h) * w + cx;
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int64_t cy = Y0 + j * (256 / kTileW) - y0;
+      if (cy >= 0 && cy < h) acc[j] = c_add(acc[j], src[cy * w]);
+    }
+  }
+  if (X >= W) return;
+#pragma unroll
+  for (int j = 0; j < RPT; ++j) {
+    const int64_t Y = Y0 + j * (256 / kTileW);
+    if (Y < H) {
+      float2* d = grad + Y * W + X;
+      *d = accumulate ? c_add(*d, acc[j]) : acc[j];
+    }
+  }
+}
+
+struct TileGrid {
+  int64_t tiles_x, tiles_y;
+  int64_t count() const { return tiles_x * tiles_y; }
+};
+inline TileGrid tile_grid(int64_t H, int64_t W) {
+  return TileGrid{(W + kTileW - 1) / kTileW, (H + kTileH - 1) / kTileH};
+}
+
+// ---------------------------------------------------------------- workspace carving
+
+struct Carver {
+  char* base;
+  size_t used;
+  template <class T>
+  T* take(size_t count) {
+    used = (used + 255) & ~size_t(255);
+    T* p = base ? reinterpret_cast<T*>(base + used) : nullptr;
+    used += count * sizeof(T);
+    return p;
+  }
+};
+
+}  // namespace ptx

@@ -1,0 +1,235 @@
+// LDS-staged Stockham FFT core for CDNA4 (gfx950), complex FP32.
+//
+// A length-N sequence is held by T = N/E threads, E complex values each, in
+// "natural" layout: thread t holds x[t + q*T] in v[q].  Each stage does E/R
+// radix-R butterflies per thread in registers; between stages the values go
+// through LDS once (ping-pong buffers, one barrier per exchange).  The first
+// stage reads, and the last stage leaves, the natural layout, so the caller's
+// global loads and stores are coalesced across the T threads of a sequence
+// (consecutive t -> consecutive addresses) and any prologue / epilogue
+// (gather, probe multiply, fftshift-by-sign, amplitude, loss gradient) is
+// applied in registers with no extra HBM round trip.
+//
+// Stage s uses radix E (or the remainder radix for the last stage) and
+// Ns = E^s; butterfly j reads x[j + m*N/R], twiddles by w_{Ns R}^{(j mod Ns) m}
+// and writes y[(j - j mod Ns) R + j mod Ns + m Ns] (Stockham autosort).
+//
+// Twiddles come from a compile-time table of exp(-2 pi i q / TW_MAX) computed
+// in double precision (Taylor series on an exact octant reduction) and rounded
+// once to float, so no fast-math sincos enters the transform.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ptx {
+
+// ---------------------------------------------------------------- complex helpers
+
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 c_mulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 c_neg(float2 a) { return make_float2(-a.x, -a.y); }
+__device__ __forceinline__ float c_abs(float2 a) { return sqrtf(a.x * a.x + a.y * a.y); }
+__device__ __forceinline__ float2 c_zero() { return make_float2(0.0f, 0.0f); }
+
+// ---------------------------------------------------------------- compile-time twiddle table
+
+constexpr int TW_MAX = 2048;  // largest supported transform length
+
+namespace detail {
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double sin_small(double x) {  // |x| <= pi/4, error < 1e-17
+  double x2 = x * x, term = x, sum = x;
+  for (int n = 1; n < 14; ++n) {
+    term *= -x2 / double((2 * n) * (2 * n + 1));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cos_small(double x) {
+  double x2 = x * x, term = 1.0, sum = 1.0;
+  for (int n = 1; n < 14; ++n) {
+    term *= -x2 / double((2 * n - 1) * (2 * n));
+    sum += term;
+  }
+  return sum;
+}
+// cos and sin of 2 pi q / M, exact octant reduction on the integer q
+constexpr void sincos_turn(int q, int M, double& c, double& s) {
+  const int quarter = M / 4, eighth = M / 8;
+  const int quad = q / quarter;
+  int r = q % quarter;
+  double cr = 0, sr = 0;
+  if (r <= eighth) {
+    const double a = 2.0 * kPi * double(r) / double(M);
+    cr = cos_small(a);
+    sr = sin_small(a);
+  } else {
+    const double a = 2.0 * kPi * double(quarter - r) / double(M);
+    cr = sin_small(a);
+    sr = cos_small(a);
+  }
+  switch (quad) {
+    case 0: c = cr; s = sr; break;
+    case 1: c = -sr; s = cr; break;
+    case 2: c = -cr; s = -sr; break;
+    default: c = sr; s = -cr; break;
+  }
+}
+struct TwiddleTable {
+  float v[2 * TW_MAX];  // interleaved (re, im) of exp(-2 pi i q / TW_MAX)
+};
+constexpr TwiddleTable make_twiddles() {
+  TwiddleTable t{};
+  for (int q = 0; q < TW_MAX; ++q) {
+    double c = 0, s = 0;
+    sincos_turn(q, TW_MAX, c, s);
+    t.v[2 * q] = float(c);
+    t.v[2 * q + 1] = float(-s);
+  }
+  return t;
+}
+// exp(-2 pi i k / 16), k = 0..7, as exact-as-float constants for the in-register DFTs
+constexpr float kC16[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
+                           0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f};
+constexpr float kS16[8] = {0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f,
+                           1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f};
+}  // namespace detail
+
+__constant__ const detail::TwiddleTable c_twiddle = detail::make_twiddles();
+
+// w^q with w = exp(-2 pi i / TW_MAX); DIR > 0 returns the conjugate (inverse transform)
+template <int DIR>
+__device__ __forceinline__ float2 twiddle(int q) {
+  const float2 w = reinterpret_cast<const float2*>(c_twiddle.v)[q];
+  return DIR > 0 ? make_float2(w.x, -w.y) : w;
+}
+
+// ---------------------------------------------------------------- in-register DFTs
+
+// multiply by exp(DIR * 2 pi i * k16 / 16) with k16 a compile-time constant after unrolling
+template <int DIR>
+__device__ __forceinline__ float2 rot16(float2 a, int k16) {
+  if (k16 == 0) return a;
+  if (k16 == 4) return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+  const float c = detail::kC16[k16];
+  const float s = DIR < 0 ? -detail::kS16[k16] : detail::kS16[k16];
+  return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+}
+
+// length-R DFT (R = 1, 2, 4, 8, 16) in natural order, sign DIR (-1 forward, +1 inverse), unnormalised
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2 (&v)[R]) {
+  if constexpr (R > 1) {
+    float2 e[R / 2], o[R / 2];
+#pragma unroll
+    for (int m = 0; m < R / 2; ++m) {
+      e[m] = v[2 * m];
+      o[m] = v[2 * m + 1];
+    }
+    dft<R / 2, DIR>(e);
+    dft<R / 2, DIR>(o);
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) {
+      const float2 t = rot16<DIR>(o[k], k * (16 / R));
+      v[k] = c_add(e[k], t);
+      v[k + R / 2] = c_sub(e[k], t);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- stage machinery
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+
+// LDS index with one pad slot per 16 entries (breaks power-of-two bank strides)
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+// per-sequence LDS length (float2 units); the +1 keeps neighbouring sequences in different banks
+template <int N>
+constexpr int lds_len() { return N == 1 ? 1 : N + N / 16 + 1; }
+
+template <int N, int E>
+struct FftShape {
+  static_assert((N & (N - 1)) == 0 && N >= 2 && N <= TW_MAX, "N must be a power of two <= TW_MAX");
+  static_assert(E == 2 || E == 4 || E == 8 || E == 16, "E must be 2, 4, 8 or 16");
+  static constexpr int T = N / E;  // threads per sequence
+  static constexpr int FULL = ilog2(N) / ilog2(E);
+  static constexpr int REM = 1 << (ilog2(N) % ilog2(E));
+  static constexpr int STAGES = FULL + (REM > 1 ? 1 : 0);
+  static constexpr bool NEEDS_LDS = STAGES > 1;
+};
+
+template <int N, int E, int DIR, int R, int NS, bool LAST, bool SYNC_FIRST>
+__device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restrict__ lds) {
+  constexpr int T = N / E;
+  constexpr int NB = E / R;               // butterflies per thread
+  constexpr int TWS = TW_MAX / (NS * R);  // twiddle stride into the table
+  // single-buffer mode: everyone must have finished reading the previous exchange
+  if constexpr (SYNC_FIRST) __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int j = t + i * T;
+    const int k = j & (NS - 1);
+    float2 a[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) a[m] = v[i + m * NB];
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int m = 1; m < R; ++m) a[m] = c_mul(a[m], twiddle<DIR>(k * m * TWS));
+    }
+    dft<R, DIR>(a);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int m = 0; m < R; ++m) v[i + m * NB] = a[m];
+    } else {
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int m = 0; m < R; ++m) lds[lds_pad(base + m * NS)] = a[m];
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = lds[lds_pad(t + q * T)];
+  }
+}
+
+// PINGPONG: alternate two LDS regions (one barrier per exchange); otherwise a
+// single region and two barriers per exchange (half the LDS footprint).
+template <int N, int E, int DIR, bool PINGPONG, int S>
+__device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, float2* lds0, float2* lds1) {
+  using Sh = FftShape<N, E>;
+  if constexpr (S < Sh::STAGES) {
+    constexpr int R = S < Sh::FULL ? E : Sh::REM;
+    constexpr int NS = ipow(E, S);
+    constexpr bool LAST = S == Sh::STAGES - 1;
+    constexpr bool SYNC_FIRST = !PINGPONG && S > 0 && !LAST;
+    fft_stage<N, E, DIR, R, NS, LAST, SYNC_FIRST>(v, t, (PINGPONG && (S & 1)) ? lds1 : lds0);
+    fft_stages<N, E, DIR, PINGPONG, S + 1>(v, t, lds0, lds1);
+  }
+}
+
+// Transform one sequence held in natural layout.  Every thread of the work-group
+// must call this uniformly (it contains work-group barriers when N > E).
+// lds0 / lds1: per-sequence regions of lds_len<N>() float2 each (lds1 unused
+// unless PINGPONG).
+template <int N, int E, int DIR, bool PINGPONG = false>
+__device__ __forceinline__ void fft_seq(float2 (&v)[E], int t, float2* lds0, float2* lds1 = nullptr) {
+  fft_stages<N, E, DIR, PINGPONG, 0>(v, t, lds0, lds1);
+}
+
+// Elements per thread used for a transform of length N.
+template <int N>
+constexpr int elems_for() { return N <= 16 ? N : 16; }
+
+}  // namespace ptx

@@ -1,0 +1,139 @@
+// Generic row / column FFT passes with pluggable prologue and epilogue.
+//
+// Row pass: a 256-thread work-group transforms S = 256/T rows of length N
+// (T = N/E threads per row), each thread loading and storing elements
+// t + q*T of its row -- fully coalesced along the contiguous axis.
+//
+// Column pass: a work-group of T*C threads transforms C adjacent columns of
+// length N; lane layout c = tid % C, t = tid / C, so every load/store
+// instruction touches C*8 contiguous bytes per row (128 B for C = 16).  The
+// whole C x N tile lives in LDS (single buffer, padded per column).
+//
+// An "op" is a plain struct of device pointers and sizes passed by value as
+// the kernel argument, with
+//   rows:    template <int N, int E> load(row, valid, t, v) / store(row, valid, t, v)
+//   columns: template <int N, int E> load(frame, col, t, v) / store(frame, col, t, v)
+// where v[q] is element t + q*(N/E) of the sequence.
+#pragma once
+
+#include "fft_core.hpp"
+
+namespace ptx {
+
+template <int N>
+struct RowCfg {
+  static constexpr int E = elems_for<N>();
+  static constexpr int T = N / E;
+  static constexpr int THREADS = 256;
+  static constexpr int S = THREADS / T;  // rows per work-group
+};
+
+template <int N>
+struct ColCfg {
+  static constexpr int E = elems_for<N>();
+  static constexpr int T = N / E;
+  // LDS budget per work-group ~72 KB (two work-groups per CU), <= 1024 threads
+  static constexpr int BY_LDS = 73728 / (8 * lds_len<N>());
+  static constexpr int BY_THREADS = 1024 / T;
+  static constexpr int CM0 = BY_LDS < BY_THREADS ? BY_LDS : BY_THREADS;
+  static constexpr int CMAX = CM0 >= 16 ? 16 : CM0 >= 8 ? 8 : CM0 >= 4 ? 4 : 2;
+};
+
+template <int N, int DIR, class Op>
+__global__ void __launch_bounds__(256) k_rows(Op op, int64_t nrows) {
+  using Cf = RowCfg<N>;
+  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  __shared__ float2 lds[S * lds_len<N>()];
+  const int seq = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int64_t row = (int64_t)blockIdx.x * S + seq;
+  const bool valid = row < nrows;
+  float2 v[E];
+  op.template load<N, E>(row, valid, t, v);
+  fft_seq<N, E, DIR>(v, t, &lds[seq * lds_len<N>()]);
+  op.template store<N, E>(row, valid, t, v);
+}
+
+template <int N, int C, int DIR, class Op>
+__global__ void __launch_bounds__(ColCfg<N>::T * C) k_cols(Op op, int64_t tiles_per_frame) {
+  using Cf = ColCfg<N>;
+  constexpr int E = Cf::E;
+  __shared__ float2 lds[C * lds_len<N>()];
+  const int c = threadIdx.x % C;
+  const int t = threadIdx.x / C;
+  const int64_t frame = blockIdx.x / tiles_per_frame;
+  const int64_t col = (blockIdx.x % tiles_per_frame) * C + c;
+  float2 v[E];
+  op.template load<N, E>(frame, col, t, v);
+  fft_seq<N, E, DIR>(v, t, &lds[c * lds_len<N>()]);
+  op.template store<N, E>(frame, col, t, v);
+}
+
+// Launch helpers (grid sizes); return false if the launch configuration is impossible.
+template <int N, int DIR, class Op>
+inline void launch_rows(const Op& op, int64_t nrows, hipStream_t stream) {
+  using Cf = RowCfg<N>;
+  const int64_t blocks = (nrows + Cf::S - 1) / Cf::S;
+  k_rows<N, DIR, Op><<<dim3((unsigned)blocks), dim3(Cf::THREADS), 0, stream>>>(op, nrows);
+}
+
+template <int N, int DIR, class Op>
+inline void launch_cols(const Op& op, int64_t frames, int64_t width, hipStream_t stream) {
+  using Cf = ColCfg<N>;
+  if (width >= Cf::CMAX) {
+    constexpr int C = Cf::CMAX;
+    const int64_t tiles = width / C;
+    k_cols<N, C, DIR, Op><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, stream>>>(op, tiles);
+  } else {
+    // narrow frames (width 8 with CMAX 16): one tile of 8 columns per frame
+    constexpr int C = 8;
+    const int64_t tiles = width / C;
+    k_cols<N, C, DIR, Op><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, stream>>>(op, tiles);
+  }
+}
+
+// ---------------------------------------------------------------- plain ops
+
+// rows of a contiguous [nrows, N] complex array, in place, scale on store
+struct RowsInPlace {
+  float2* data;
+  float scale;
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    const float2* p = data + row * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = valid ? p[t + q * T] : c_zero();
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    if (!valid) return;
+    float2* p = data + row * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) p[t + q * T] = c_scale(v[q], scale);
+  }
+};
+
+// columns of contiguous [frames, N, width] complex frames, in place, scale on store
+struct ColsInPlace {
+  float2* data;
+  int64_t width;
+  float scale;
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    const float2* p = data + frame * N * width + col;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = p[(int64_t)(t + q * T) * width];
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    float2* p = data + frame * N * width + col;
+#pragma unroll
+    for (int q = 0; q < E; ++q) p[(int64_t)(t + q * T) * width] = c_scale(v[q], scale);
+  }
+};
+
+}  // namespace ptx

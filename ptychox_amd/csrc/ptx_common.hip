@@ -1,0 +1,83 @@
+// Error reporting, version queries and the standalone batched 2-D FFT entry point.
+#include <cstdarg>
+#include <cstdio>
+
+#include "passes.hpp"
+#include "ptx_internal.hpp"
+
+namespace ptx {
+
+static thread_local char g_last_error[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+}
+
+ptx_status check_launch(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("%s: HIP launch failed: %s", what, hipGetErrorString(e));
+    return PTX_ERR_HIP;
+  }
+  return PTX_OK;
+}
+
+template <int N>
+struct FftRowsRun {
+  static ptx_status run(float2* data, int64_t nrows, int inverse, hipStream_t s) {
+    RowsInPlace op{data, 1.0f};
+    if (inverse)
+      launch_rows<N, +1>(op, nrows, s);
+    else
+      launch_rows<N, -1>(op, nrows, s);
+    return check_launch("fft2d rows");
+  }
+};
+
+template <int N>
+struct FftColsRun {
+  static ptx_status run(float2* data, int64_t frames, int64_t width, int inverse, hipStream_t s) {
+    ColsInPlace op{data, width, 1.0f};
+    if (inverse)
+      launch_cols<N, +1>(op, frames, width, s);
+    else
+      launch_cols<N, -1>(op, frames, width, s);
+    return check_launch("fft2d cols");
+  }
+};
+
+}  // namespace ptx
+
+using namespace ptx;
+
+extern "C" {
+
+const char* ptx_last_error(void) { return g_last_error; }
+
+const char* ptx_version(void) { return "ptychox_hip 0.1.0 (gfx950)"; }
+
+int ptx_abi_version(void) { return PTX_ABI_VERSION; }
+
+ptx_status ptx_fft2d(void* data, int64_t batch, int64_t h, int64_t w, int inverse, void* stream) {
+  g_last_error[0] = 0;
+  if (batch < 0 || (batch > 0 && data == nullptr)) {
+    set_error("ptx_fft2d: invalid data pointer or batch");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  if (!size_ok(h) || !size_ok(w)) {
+    set_error("ptx_fft2d: frame %lld x %lld unsupported (powers of two in [%d, %d])", (long long)h, (long long)w,
+              PTX_MIN_N, PTX_MAX_N);
+    return PTX_ERR_UNSUPPORTED;
+  }
+  if (batch == 0) return PTX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  float2* d = (float2*)data;
+  ptx_status st = dispatch_n<FftRowsRun>(w, d, batch * h, inverse, s);
+  if (st != PTX_OK) return st;
+  return dispatch_n<FftColsRun>(h, d, batch, w, inverse, s);
+}
+
+}  // extern "C"
