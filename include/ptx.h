@@ -58,6 +58,19 @@ const char* ptx_last_error(void);
 const char* ptx_version(void);
 int ptx_abi_version(void);
 
+/* ------------------------------------------------------------------ profiling hook
+ * No counterpart in the reference (it only times whole epochs with
+ * torch.cuda.Event, python/train_fpm.py:237-239,275-277).  Arms per-kernel
+ * timing for the following calls on this thread: each entry point records the
+ * next event of ``events`` on its stream before its first kernel (label
+ * "begin") and one after each of its kernels (label = the kernel), until n
+ * events are used; ptx_timing_label(k) names the interval that ends at
+ * events[k].  events are hipEvent_t handles owned by the caller and must stay
+ * valid while armed; n = 0 disarms.  Costs nothing when disarmed. */
+ptx_status ptx_timing_events(void* const* events, int n);
+int ptx_timing_count(void);
+const char* ptx_timing_label(int k);
+
 /* ------------------------------------------------------------------ FFT
  * Batched in-place 2-D C2C transform of ``batch`` frames of h x w,
  * unnormalised in both directions (inverse != 0 uses exp(+i)).

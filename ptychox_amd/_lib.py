@@ -40,6 +40,9 @@ SIGNATURES: dict[str, tuple] = {
     "ptx_last_error": (ctypes.c_char_p, []),
     "ptx_version": (ctypes.c_char_p, []),
     "ptx_abi_version": (_i, []),
+    "ptx_timing_events": (_st, [_p, _i]),
+    "ptx_timing_count": (_i, []),
+    "ptx_timing_label": (ctypes.c_char_p, [_i]),
     "ptx_fft2d": (_st, [_p, _i64, _i64, _i64, _i, _p]),
     "ptx_base_forward": (_st, [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _p, _p]),
     "ptx_base_backward_workspace": (_sz, [_i64, _i64, _i64, _i64, _i64, _u]),
@@ -113,3 +116,38 @@ def stream_of(device: torch.device) -> ctypes.c_void_p:
 
 def workspace(nbytes: int, device: torch.device) -> torch.Tensor:
     return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+class KernelTimer:
+    """Per-kernel durations of the next entry-point calls, from hipEvents the
+    library records on the call's stream between its kernels
+    (``ptx_timing_events``).  Use::
+
+        with KernelTimer(n_events=64) as kt:
+            step(...)
+        kt.durations()   # [(label, ms), ...] in launch order
+    """
+
+    def __init__(self, n_events: int = 64):
+        self.events = [torch.cuda.Event(enable_timing=True) for _ in range(n_events)]
+        for e in self.events:  # materialise the hipEvent_t handles
+            e.record()
+        torch.cuda.synchronize()
+        self._arr = (ctypes.c_void_p * n_events)(*[e.cuda_event for e in self.events])
+
+    def __enter__(self):
+        call("ptx_timing_events", ctypes.cast(self._arr, ctypes.c_void_p), len(self.events))
+        return self
+
+    def __exit__(self, *exc):
+        self.count = lib().ptx_timing_count()
+        self.labels = [lib().ptx_timing_label(k).decode() for k in range(self.count)]
+        call("ptx_timing_events", None, 0)
+        return False
+
+    def durations(self) -> list[tuple[str, float]]:
+        torch.cuda.synchronize()
+        return [
+            (self.labels[k], self.events[k - 1].elapsed_time(self.events[k]))
+            for k in range(1, self.count)
+        ]

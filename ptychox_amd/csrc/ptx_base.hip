@@ -206,21 +206,21 @@ template <int N>
 struct GatherRowsRun {
   static ptx_status run(const BaseGatherRows& op, int64_t nrows, hipStream_t s) {
     launch_rows<N, -1>(op, nrows, s);
-    return check_launch("base gather rows");
+    return check_launch("base gather rows", s);
   }
 };
 template <int N>
 struct FarFieldColsRun {
   static ptx_status run(const BaseFarFieldCols& op, int64_t frames, hipStream_t s) {
     launch_cols<N, -1>(op, frames, op.width, s);
-    return check_launch("base far-field cols");
+    return check_launch("base far-field cols", s);
   }
 };
 template <int N>
 struct GammaColsRun {
   static ptx_status run(const BaseGammaCols& op, int64_t frames, hipStream_t s) {
     launch_cols<N, +1>(op, frames, op.width, s);
-    return check_launch("base gamma cols");
+    return check_launch("base gamma cols", s);
   }
 };
 
@@ -241,7 +241,7 @@ struct StepColsRun {
       k_base_step_cols<N, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(tb, measured, amp, loss_part,
                                                                                             width, tiles, gscale);
     }
-    return check_launch("base step cols");
+    return check_launch("base step cols", s);
   }
 };
 
@@ -285,7 +285,7 @@ struct ScatterRowsRun {
                         float2* gp_part, hipStream_t s) {
     k_base_scatter_rows<N><<<dim3((unsigned)(p.ngroups * p.nchunks)), dim3(256), 0, s>>>(
         tb, sample, H, W, probe, h, pos, B, p.chunk, p.ngroups, gobj, contrib, gp_part);
-    return check_launch("base scatter rows");
+    return check_launch("base scatter rows", s);
   }
 };
 
@@ -338,19 +338,26 @@ ptx_status base_scatter(const BaseWorkspace& ws, const ScatterPlan& sp, const fl
                         float2* gp, unsigned flags, hipStream_t s) {
   const bool det = flags & PTX_FLAG_DETERMINISTIC;
   const bool acc = flags & PTX_FLAG_ACCUMULATE;
-  if (!det && !acc) fill_zero2(gs, H * W, s);
-  ptx_status st = dispatch_n<ScatterRowsRun>(w, ws.tb, sample, H, W, probe, (int)h, pos, B, sp, gs,
-                                             det ? ws.tb : nullptr, ws.gp_part, s);
+  ptx_status st = PTX_OK;
+  if (!det && !acc) {
+    fill_zero2(gs, H * W, s);
+    if ((st = check_launch("zero fill", s)) != PTX_OK) return st;
+  }
+  st = dispatch_n<ScatterRowsRun>(w, ws.tb, sample, H, W, probe, (int)h, pos, B, sp, gs, det ? ws.tb : nullptr,
+                                  ws.gp_part, s);
   if (st != PTX_OK) return st;
   sum_parts2(ws.gp_part, sp.nchunks, h * w, gp, acc ? 1 : 0, s);
+  if ((st = check_launch("probe partial sum", s)) != PTX_OK) return st;
   if (det) {
     const TileGrid tg = tile_grid(H, W);
     k_tile_lists<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(pos, B, (int)h, (int)w, tg.tiles_x, ws.lists,
                                                                    ws.counts);
+    if ((st = check_launch("tile lists", s)) != PTX_OK) return st;
     k_tile_gather<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(ws.tb, (int)h, (int)w, pos, B, ws.lists, ws.counts,
                                                                     tg.tiles_x, gs, H, W, acc ? 1 : 0);
+    if ((st = check_launch("tile gather", s)) != PTX_OK) return st;
   }
-  return check_launch("base scatter tail");
+  return PTX_OK;
 }
 
 }  // namespace
@@ -370,6 +377,7 @@ ptx_status ptx_base_forward(const void* sample, int64_t H, int64_t W, const void
     return PTX_ERR_INVALID_ARGUMENT;
   }
   hipStream_t s = (hipStream_t)stream;
+  timing_begin(s);
   BaseGatherRows r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, pos, (float2*)far_field};
   st = dispatch_n<GatherRowsRun>(w, r1, B * h, s);
   if (st != PTX_OK) return st;
@@ -399,7 +407,7 @@ ptx_status ptx_base_backward(const float* grad_output, const void* far_field, co
       fill_zero2((float2*)grad_sample, H * W, s);
       fill_zero2((float2*)grad_probe, h * w, s);
     }
-    return check_launch("ptx_base_backward");
+    return check_launch("zero fill", s);
   }
   if (!grad_output || !far_field || !sample || !probe || !pos) {
     set_error("ptx_base_backward: null pointer argument");
@@ -412,6 +420,7 @@ ptx_status ptx_base_backward(const float* grad_output, const void* far_field, co
     set_error("ptx_base_backward: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
     return PTX_ERR_WORKSPACE;
   }
+  timing_begin(s);
   BaseGammaCols c2{(const float2*)far_field, grad_output, ws.tb, w};
   st = dispatch_n<GammaColsRun>(h, c2, B, s);
   if (st != PTX_OK) return st;
@@ -444,7 +453,7 @@ ptx_status ptx_base_step(const void* sample, int64_t H, int64_t W, const void* p
       fill_zero2((float2*)grad_sample, H * W, s);
       fill_zero2((float2*)grad_probe, h * w, s);
     }
-    return check_launch("ptx_base_step");
+    return check_launch("zero fill", s);
   }
   if (!sample || !probe || !pos || !measured || !loss_per_position) {
     set_error("ptx_base_step: null pointer argument");
@@ -459,12 +468,14 @@ ptx_status ptx_base_step(const void* sample, int64_t H, int64_t W, const void* p
     set_error("ptx_base_step: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
     return PTX_ERR_WORKSPACE;
   }
+  timing_begin(s);
   BaseGatherRows r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, pos, ws.tb};
   st = dispatch_n<GatherRowsRun>(w, r1, B * h, s);
   if (st != PTX_OK) return st;
   st = dispatch_n<StepColsRun>(h, ws.tb, measured, amplitude, ws.loss_part, B, w, 2.0f * loss_scale, s);
   if (st != PTX_OK) return st;
   loss_rows(ws.loss_part, tiles, B, 1.0f / (float)(h * w), loss_per_position, s);
+  if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
   return base_scatter(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, pos, B, (float2*)grad_sample,
                       (float2*)grad_probe, flags, s);
 }

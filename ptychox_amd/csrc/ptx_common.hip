@@ -16,12 +16,30 @@ void set_error(const char* fmt, ...) {
   va_end(ap);
 }
 
-ptx_status check_launch(const char* what) {
+static thread_local hipEvent_t* t_events = nullptr;
+static thread_local int t_nevents = 0;
+static thread_local int t_used = 0;
+static thread_local const char* t_labels[256];
+
+static void timing_record(hipStream_t stream, const char* label) {
+  if (t_used < t_nevents && t_used < 256) {
+    hipEventRecord(t_events[t_used], stream);
+    t_labels[t_used] = label;
+    ++t_used;
+  }
+}
+
+void timing_begin(hipStream_t stream) {
+  if (t_nevents > 0) timing_record(stream, "begin");
+}
+
+ptx_status check_launch(const char* what, hipStream_t stream) {
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("%s: HIP launch failed: %s", what, hipGetErrorString(e));
     return PTX_ERR_HIP;
   }
+  if (t_nevents > 0) timing_record(stream, what);
   return PTX_OK;
 }
 
@@ -33,7 +51,7 @@ struct FftRowsRun {
       launch_rows<N, +1>(op, nrows, s);
     else
       launch_rows<N, -1>(op, nrows, s);
-    return check_launch("fft2d rows");
+    return check_launch("fft2d rows", s);
   }
 };
 
@@ -45,7 +63,7 @@ struct FftColsRun {
       launch_cols<N, +1>(op, frames, width, s);
     else
       launch_cols<N, -1>(op, frames, width, s);
-    return check_launch("fft2d cols");
+    return check_launch("fft2d cols", s);
   }
 };
 
@@ -56,6 +74,21 @@ using namespace ptx;
 extern "C" {
 
 const char* ptx_last_error(void) { return g_last_error; }
+
+ptx_status ptx_timing_events(void* const* events, int n) {
+  if (n < 0 || (n > 0 && events == nullptr)) {
+    set_error("ptx_timing_events: invalid event array");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  t_events = n > 0 ? (hipEvent_t*)events : nullptr;
+  t_nevents = n > 256 ? 256 : n;
+  t_used = 0;
+  return PTX_OK;
+}
+
+int ptx_timing_count(void) { return t_used; }
+
+const char* ptx_timing_label(int k) { return (k >= 0 && k < t_used) ? t_labels[k] : ""; }
 
 const char* ptx_version(void) { return "ptychox_hip 0.1.0 (gfx950)"; }
 
@@ -74,6 +107,7 @@ ptx_status ptx_fft2d(void* data, int64_t batch, int64_t h, int64_t w, int invers
   }
   if (batch == 0) return PTX_OK;
   hipStream_t s = (hipStream_t)stream;
+  timing_begin(s);
   float2* d = (float2*)data;
   ptx_status st = dispatch_n<FftRowsRun>(w, d, batch * h, inverse, s);
   if (st != PTX_OK) return st;

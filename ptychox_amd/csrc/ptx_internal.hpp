@@ -20,7 +20,12 @@ inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 inline bool size_ok(int64_t n) { return is_pow2(n) && n >= PTX_MIN_N && n <= PTX_MAX_N; }
 
 // Returns PTX_ERR_HIP (with the message recorded) if the last launch failed.
-ptx_status check_launch(const char* what);
+// Also records the next armed timing event (ptx_timing_events) after the
+// kernel(s) named `what`.
+ptx_status check_launch(const char* what, hipStream_t stream = nullptr);
+
+// Records timing event 0 when armed; called by each entry point before its first kernel.
+void timing_begin(hipStream_t stream);
 
 // Call F<N>::run(args...) for the runtime power-of-two n.
 template <template <int> class F, typename... Args>

@@ -1,0 +1,87 @@
+"""CPU-side checks of the C ABI: the library loads, exports every entry point
+``include/ptx.h`` declares, and rejects bad arguments before touching a GPU."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols() -> set[str]:
+    names = set()
+    for hdr in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", hdr.read_text(), flags=re.S)
+        names |= set(re.findall(r"\b(ptx_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from ptychox_amd import _lib
+
+    return _lib
+
+
+def test_library_exports_every_declared_symbol(lib):
+    handle = lib.lib()
+    declared = declared_symbols()
+    assert declared, "no declarations found"
+    missing = [n for n in sorted(declared) if not hasattr(handle, n)]
+    assert not missing, missing
+    # and the ctypes table covers exactly the declared surface
+    assert set(lib.SIGNATURES) == declared
+
+
+def test_dynamic_symbols_are_the_c_abi_only():
+    so = ROOT / "ptychox_amd" / "libptychox_hip.so"
+    out = subprocess.run(["nm", "-D", "--defined-only", str(so)], capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    ours = {s for s in exported if s.startswith("ptx_")}
+    assert ours == declared_symbols()
+
+
+def test_version(lib):
+    h = lib.lib()
+    assert h.ptx_abi_version() == 1
+    assert b"gfx950" in h.ptx_version()
+
+
+def test_unsupported_sizes_are_rejected_without_gpu(lib):
+    h = lib.lib()
+    st = h.ptx_fft2d(ctypes.c_void_p(1), 1, 12, 16, 0, None)
+    assert st == 2 and b"unsupported" in h.ptx_last_error()
+    st = h.ptx_fft2d(ctypes.c_void_p(1), 1, 16, 4096, 0, None)
+    assert st == 2
+    st = h.ptx_base_forward(None, 64, 64, None, 24, 24, None, 1, None, None, None)
+    assert st == 2
+    st = h.ptx_base_forward(None, 16, 16, None, 32, 32, None, 1, None, None, None)
+    assert st == 3  # probe larger than the sample
+    st = h.ptx_base_forward(None, 64, 64, None, 16, 16, None, 1, None, None, None)
+    assert st == 1 and b"null" in h.ptx_last_error()
+
+
+def test_workspace_queries(lib):
+    n = lib.query("ptx_base_backward_workspace", 2048, 2048, 512, 512, 64, 0)
+    assert n >= 64 * 512 * 512 * 8
+    n_det = lib.query("ptx_base_backward_workspace", 2048, 2048, 512, 512, 64, 1)
+    assert n_det > n
+    assert lib.query("ptx_base_step_workspace", 2048, 2048, 512, 512, 64, 0) > n
+    assert lib.query("ptx_base_backward_workspace", 2048, 2048, 500, 512, 64, 0) == 0
+
+
+def test_call_raises_with_message(lib):
+    with pytest.raises(ValueError, match="null pointer"):
+        lib.call("ptx_base_forward", None, 64, 64, None, 16, 16, None, 1, None, None, None)
+
+
+def test_timing_hook_arms_and_disarms(lib):
+    lib.call("ptx_timing_events", None, 0)
+    assert lib.lib().ptx_timing_count() == 0
+    with pytest.raises(ValueError):
+        lib.call("ptx_timing_events", None, 4)
