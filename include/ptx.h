@@ -130,6 +130,88 @@ ptx_status ptx_base_step(const void* sample, int64_t sample_h, int64_t sample_w,
                          void* grad_sample, void* grad_probe,
                          void* workspace, size_t workspace_bytes, unsigned flags, void* stream);
 
+/* ------------------------------------------------------------------ Fourier ptychography
+ * Forward of ptychoX_fourier: object_fft = fft2(object_field);
+ * sub_b[r,c] = object_fft[(y_b + r + H/2) mod H, (x_b + c + W/2) mod W];
+ * image_field[b] = ifft2(fftshift(sub_b * pupil)); amplitude = |image_field|.
+ * led_indices_xy may be negative (wrap-around, python/ptychoX/fpm/ops.py:122-123).
+ * Replaces ptychox_fpm_forward_cuda (python/ptychoX/csrc/fpm_kernel.cu:233-273, bound at
+ * fpm_extension.cpp:21-32) and the torch branch fpm/ops.py:56-59.  The reference
+ * returns the gathered sub-spectra; this library re-gathers them from object_fft.
+ * object_fft: [H, W] complex64 out; image_field: [batch, h, w] complex64 out;
+ * amplitude: [batch, h, w] float out. */
+ptx_status ptx_fpm_forward(const void* object_field, int64_t object_h, int64_t object_w,
+                           const void* pupil, int64_t pupil_h, int64_t pupil_w,
+                           const int64_t* led_indices_xy, int64_t batch,
+                           void* object_fft, void* image_field, float* amplitude, void* stream);
+
+size_t ptx_fpm_backward_workspace(int64_t object_h, int64_t object_w, int64_t pupil_h, int64_t pupil_w,
+                                  int64_t batch, unsigned flags);
+
+/* Backward of ptychoX_fourier: g_b = ifftshift(fft2(G E/max(|E|, FLT_EPSILON)))/(h w);
+ * grad_pupil = sum_b g_b conj(sub_b); grad_object = H W ifft2(index_add(g_b conj(pupil))).
+ * Replaces ptychox_fpm_backward_cuda (python/ptychoX/csrc/fpm_kernel.cu:275-317, bound at
+ * fpm_extension.cpp:34-60) and the torch branch fpm/ops.py:85-97.
+ * grad_object: [H, W] complex64; grad_pupil: [h, w] complex64. */
+ptx_status ptx_fpm_backward(const float* grad_output, const void* image_field, const void* object_fft,
+                            int64_t object_h, int64_t object_w,
+                            const void* pupil, int64_t pupil_h, int64_t pupil_w,
+                            const int64_t* led_indices_xy, int64_t batch,
+                            void* grad_object, void* grad_pupil,
+                            void* workspace, size_t workspace_bytes, unsigned flags, void* stream);
+
+size_t ptx_fpm_step_workspace(int64_t object_h, int64_t object_w, int64_t pupil_h, int64_t pupil_w,
+                              int64_t batch, unsigned flags);
+
+/* Fused FPM training step with the amplitude MSE loss of python/train_fpm.py:257-263
+ * (same loss conventions as ptx_base_step). */
+ptx_status ptx_fpm_step(const void* object_field, int64_t object_h, int64_t object_w,
+                        const void* pupil, int64_t pupil_h, int64_t pupil_w,
+                        const int64_t* led_indices_xy, int64_t batch,
+                        const float* measured, float loss_scale,
+                        float* loss_per_position, float* amplitude,
+                        void* grad_object, void* grad_pupil,
+                        void* workspace, size_t workspace_bytes, unsigned flags, void* stream);
+
+/* ------------------------------------------------------------------ near-field
+ * Forward of ptychoX_nearfield: field[b] = ifft2(transfer * fft2(sample[y_b:y_b+h, x_b:x_b+w] * probe)),
+ * amplitude = |field|; transfer is [h, w] complex64 with DC at [0, 0] (angular
+ * spectrum exp(i pi lambda z (fx^2 + fy^2)) for example).
+ * Replaces the MATLAB near-field operator's propagation chain
+ * (matlab/PtychoX_updated/scripts/forward/fwd_adPtychoX_nearf.cu:58-114), applied
+ * per scan window; the reference has no Python near-field operator. */
+ptx_status ptx_nearfield_forward(const void* sample, int64_t sample_h, int64_t sample_w,
+                                 const void* probe, int64_t probe_h, int64_t probe_w, const void* transfer,
+                                 const int64_t* positions_xy, int64_t batch,
+                                 void* field, float* amplitude, void* stream);
+
+size_t ptx_nearfield_backward_workspace(int64_t sample_h, int64_t sample_w, int64_t probe_h, int64_t probe_w,
+                                        int64_t batch, unsigned flags);
+
+/* Backward: chi_b = ifft2(conj(transfer) fft2(G field/max(|field|, FLT_EPSILON)));
+ * grad_sample[win_b] += chi_b conj(probe); grad_probe += sum_b chi_b conj(patch_b).
+ * Replaces matlab/PtychoX_updated/scripts/backward/bwd_adPtychoX_nearf.cu:81-144
+ * (whose gradient is the heuristic dl/(out + 0.001), kernels_nearf.cuh:525;
+ * this one is the exact adjoint). */
+ptx_status ptx_nearfield_backward(const float* grad_output, const void* field,
+                                  const void* sample, int64_t sample_h, int64_t sample_w,
+                                  const void* probe, int64_t probe_h, int64_t probe_w, const void* transfer,
+                                  const int64_t* positions_xy, int64_t batch,
+                                  void* grad_sample, void* grad_probe,
+                                  void* workspace, size_t workspace_bytes, unsigned flags, void* stream);
+
+size_t ptx_nearfield_step_workspace(int64_t sample_h, int64_t sample_w, int64_t probe_h, int64_t probe_w,
+                                    int64_t batch, unsigned flags);
+
+/* Fused near-field training step with the amplitude MSE loss (conventions of ptx_base_step). */
+ptx_status ptx_nearfield_step(const void* sample, int64_t sample_h, int64_t sample_w,
+                              const void* probe, int64_t probe_h, int64_t probe_w, const void* transfer,
+                              const int64_t* positions_xy, int64_t batch,
+                              const float* measured, float loss_scale,
+                              float* loss_per_position, float* amplitude,
+                              void* grad_sample, void* grad_probe,
+                              void* workspace, size_t workspace_bytes, unsigned flags, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

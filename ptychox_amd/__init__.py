@@ -9,16 +9,29 @@ the hand-written HIP kernels of ``libptychox_hip.so``.
 from ._utils import matlab_positions_to_torch, set_deterministic
 from .base import clear_cuda_plan_cache as clear_base_cuda_plan_cache
 from .base import ptychoX_base, ptychography_base
+from .fpm import clear_cuda_plan_cache as clear_fpm_cuda_plan_cache
+from .fpm import fourier_ptychography, ptychoX_fourier
+from .nearfield import angular_spectrum_transfer, nearfield_ptychography, ptychoX_nearfield
+from .nearfield import clear_cuda_plan_cache as clear_nearfield_cuda_plan_cache
 
 
 def clear_cuda_plan_cache() -> None:
     """Kept for drop-in compatibility (python/ptychoX/__init__.py:10-13)."""
     clear_base_cuda_plan_cache()
+    clear_fpm_cuda_plan_cache()
+    clear_nearfield_cuda_plan_cache()
 
 
 __all__ = [
     "clear_cuda_plan_cache",
     "clear_base_cuda_plan_cache",
+    "clear_fpm_cuda_plan_cache",
+    "clear_nearfield_cuda_plan_cache",
+    "angular_spectrum_transfer",
+    "nearfield_ptychography",
+    "ptychoX_nearfield",
+    "fourier_ptychography",
+    "ptychoX_fourier",
     "matlab_positions_to_torch",
     "ptychoX_base",
     "ptychography_base",

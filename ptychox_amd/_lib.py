@@ -55,6 +55,28 @@ SIGNATURES: dict[str, tuple] = {
         _st,
         [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _f, _p, _p, _p, _p, _p, _sz, _u, _p],
     ),
+    "ptx_fpm_forward": (_st, [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _p, _p, _p]),
+    "ptx_fpm_backward_workspace": (_sz, [_i64, _i64, _i64, _i64, _i64, _u]),
+    "ptx_fpm_backward": (
+        _st,
+        [_p, _p, _p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _p, _p, _sz, _u, _p],
+    ),
+    "ptx_fpm_step_workspace": (_sz, [_i64, _i64, _i64, _i64, _i64, _u]),
+    "ptx_fpm_step": (
+        _st,
+        [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _f, _p, _p, _p, _p, _p, _sz, _u, _p],
+    ),
+    "ptx_nearfield_forward": (_st, [_p, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _p, _p, _p]),
+    "ptx_nearfield_backward_workspace": (_sz, [_i64, _i64, _i64, _i64, _i64, _u]),
+    "ptx_nearfield_backward": (
+        _st,
+        [_p, _p, _p, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _p, _p, _p, _sz, _u, _p],
+    ),
+    "ptx_nearfield_step_workspace": (_sz, [_i64, _i64, _i64, _i64, _i64, _u]),
+    "ptx_nearfield_step": (
+        _st,
+        [_p, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _p, _f, _p, _p, _p, _p, _p, _sz, _u, _p],
+    ),
 }
 
 _lock = threading.Lock()

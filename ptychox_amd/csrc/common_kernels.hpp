@@ -70,11 +70,23 @@ __device__ __forceinline__ float block_sum(float x, float* red /* >= 16 floats o
 constexpr int kTileH = 16;   // object rows per gather tile
 constexpr int kTileW = 128;  // object columns per gather tile
 
-// For every object tile, the views whose h x w window overlaps it, in ascending
-// view order.  lists: [ntiles, batch] int32; counts: [ntiles].
+__device__ __forceinline__ int64_t pmod(int64_t a, int64_t n) {
+  const int64_t r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+// Windows are h x w rectangles whose top-left corner is
+//   ((y_b + off_y) mod H, (x_b + off_x) mod W)
+// on an H x W grid, wrapping around its edges: off = 0 gives the conventional
+// scan window (never wraps for valid positions), off = (H/2, W/2) the Fourier
+// ptychography sub-spectrum of python/ptychoX/fpm/ops.py:113-124.
+
+// For every grid tile, the views whose window overlaps it, in ascending view
+// order.  lists: [ntiles, batch] int32; counts: [ntiles].
 static __global__ void __launch_bounds__(256) k_tile_lists(const int64_t* __restrict__ pos, int64_t batch, int h, int w,
-                                                    int64_t tiles_x, int* __restrict__ lists,
-                                                    int* __restrict__ counts) {
+                                                           int64_t H, int64_t W, int64_t off_y, int64_t off_x,
+                                                           int64_t tiles_x, int* __restrict__ lists,
+                                                           int* __restrict__ counts) {
   __shared__ int wave_tot[4];
   const int64_t tile = blockIdx.x;
   const int64_t ty0 = (tile / tiles_x) * kTileH, tx0 = (tile % tiles_x) * kTileW;
@@ -84,8 +96,11 @@ static __global__ void __launch_bounds__(256) k_tile_lists(const int64_t* __rest
     const int64_t b = base + threadIdx.x;
     bool hit = false;
     if (b < batch) {
-      const int64_t x0 = pos[2 * b], y0 = pos[2 * b + 1];
-      hit = y0 < ty0 + kTileH && y0 + h > ty0 && x0 < tx0 + kTileW && x0 + w > tx0;
+      const int64_t oy = pmod(pos[2 * b + 1] + off_y, H), ox = pmod(pos[2 * b] + off_x, W);
+      // two arcs on a circle overlap iff one starts inside the other
+      const bool ry = pmod(ty0 - oy, H) < h || pmod(oy - ty0, H) < kTileH;
+      const bool rx = pmod(tx0 - ox, W) < w || pmod(ox - tx0, W) < kTileW;
+      hit = ry && rx;
     }
     const unsigned long long m = __ballot(hit);
     const int before = __popcll(m & ((1ull << lane) - 1ull));
@@ -100,13 +115,17 @@ static __global__ void __launch_bounds__(256) k_tile_lists(const int64_t* __rest
   if (threadIdx.x == 0) counts[tile] = count;
 }
 
-// grad[Y, X] (+)= sum over listed views b (ascending) of contrib[b, Y - y_b, X - x_b]
+// grad[Y, X] (+)= sum over listed views b (ascending) of contrib[b, cy, cx] with
+// (cy, cx) = ((Y - oy_b) mod H, (X - ox_b) mod W) inside the h x w window.
 static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __restrict__ contrib, int h, int w,
-                                                     const int64_t* __restrict__ pos, int64_t batch,
-                                                     const int* __restrict__ lists, const int* __restrict__ counts,
-                                                     int64_t tiles_x, float2* __restrict__ grad, int64_t H, int64_t W,
-                                                     int accumulate) {
+                                                            const int64_t* __restrict__ pos, int64_t batch,
+                                                            int64_t off_y, int64_t off_x,
+                                                            const int* __restrict__ lists,
+                                                            const int* __restrict__ counts, int64_t tiles_x,
+                                                            float2* __restrict__ grad, int64_t H, int64_t W,
+                                                            int accumulate) {
   constexpr int RPT = kTileH * kTileW / 256;  // rows per thread
+  constexpr int RSTEP = 256 / kTileW;
   const int64_t tile = blockIdx.x;
   const int64_t ty0 = (tile / tiles_x) * kTileH, tx0 = (tile % tiles_x) * kTileW;
   const int64_t X = tx0 + (threadIdx.x % kTileW);
@@ -117,20 +136,21 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
   const int n = counts[tile];
   for (int k = 0; k < n; ++k) {
     const int64_t b = lists[tile * batch + k];
-    const int64_t x0 = pos[2 * b], y0 = pos[2 * b + 1];
-    const int64_t cx = X - x0;
-    if (cx < 0 || cx >= w) continue;
+    const int64_t oy = pmod(pos[2 * b + 1] + off_y, H), ox = pmod(pos[2 * b] + off_x, W);
+    const int64_t cx = pmod(X - ox, W);
+    if (X >= W || cx >= w) continue;
     const float2* src = contrib + (b * h) * w + cx;
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
-      const int64_t cy = Y0 + j * (256 / kTileW) - y0;
-      if (cy >= 0 && cy < h) acc[j] = c_add(acc[j], src[cy * w]);
+      const int64_t Y = Y0 + j * RSTEP;
+      const int64_t cy = pmod(Y - oy, H);
+      if (Y < H && cy < h) acc[j] = c_add(acc[j], src[cy * w]);
     }
   }
   if (X >= W) return;
 #pragma unroll
   for (int j = 0; j < RPT; ++j) {
-    const int64_t Y = Y0 + j * (256 / kTileW);
+    const int64_t Y = Y0 + j * RSTEP;
     if (Y < H) {
       float2* d = grad + Y * W + X;
       *d = accumulate ? c_add(*d, acc[j]) : acc[j];

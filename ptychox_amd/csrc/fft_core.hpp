@@ -184,8 +184,15 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restr
 #pragma unroll
     for (int m = 0; m < R; ++m) a[m] = v[i + m * NB];
     if constexpr (NS > 1) {
+      // w^(k m) by recurrence from one table load: relative error grows by
+      // ~1 ulp per step (< 1e-6 for R = 16) and saves R - 2 memory loads
+      const float2 w1 = twiddle<DIR>(k * TWS);
+      float2 wm = w1;
 #pragma unroll
-      for (int m = 1; m < R; ++m) a[m] = c_mul(a[m], twiddle<DIR>(k * m * TWS));
+      for (int m = 1; m < R; ++m) {
+        a[m] = c_mul(a[m], wm);
+        if (m + 1 < R) wm = c_mul(wm, w1);
+      }
     }
     dft<R, DIR>(a);
     if constexpr (LAST) {

@@ -115,6 +115,28 @@ struct RowsInPlace {
   }
 };
 
+// rows of a contiguous [nrows, N] complex array from `in` to `out`, scale on store
+struct RowsOutOfPlace {
+  const float2* in;
+  float2* out;
+  float scale;
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    const float2* p = in + row * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = valid ? p[t + q * T] : c_zero();
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    if (!valid) return;
+    float2* p = out + row * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) p[t + q * T] = c_scale(v[q], scale);
+  }
+};
+
 // columns of contiguous [frames, N, width] complex frames, in place, scale on store
 struct ColsInPlace {
   float2* data;

@@ -15,46 +15,10 @@
 //             R2 rows  : row iFFT, sign, chi conj(p) -> object (atomics or per-view buffer),
 //                        chi conj(patch) accumulated over a chunk of views in registers -> probe partials
 //   step:     R1, then C12 cols: column FFT, |Phi|, loss partials, Gamma, column iFFT (in place), then R2.
-#include "common_kernels.hpp"
-#include "passes.hpp"
-#include "ptx_internal.hpp"
+#include "scan_common.hpp"
 
 namespace ptx {
 namespace {
-
-// ---------------------------------------------------------------- R1: gather * probe * sign, row FFT
-struct BaseGatherRows {
-  const float2* sample;
-  int64_t H, W;
-  const float2* probe;
-  int h;
-  const int64_t* pos;
-  float2* out;  // [B, h, w]
-  template <int N, int E>
-  __device__ __forceinline__ void load(int64_t row, bool valid, int t, float2 (&v)[E]) const {
-    constexpr int T = N / E;
-    const int64_t b = valid ? row / h : 0;
-    const int r = valid ? (int)(row - b * h) : 0;
-    const int64_t x0 = pos[2 * b], y0 = pos[2 * b + 1];
-    const bool ok = valid && window_ok(x0, y0, h, N, H, W);
-    const float2* srow = sample + (y0 + r) * W + x0;
-    const float2* prow = probe + (int64_t)r * N;
-#pragma unroll
-    for (int q = 0; q < E; ++q) {
-      const int i = t + q * T;
-      float2 e = ok ? c_mul(srow[i], prow[i]) : c_zero();
-      v[q] = ((r + i) & 1) ? c_neg(e) : e;
-    }
-  }
-  template <int N, int E>
-  __device__ __forceinline__ void store(int64_t row, bool valid, int t, float2 (&v)[E]) const {
-    constexpr int T = N / E;
-    if (!valid) return;
-    float2* d = out + row * N;
-#pragma unroll
-    for (int q = 0; q < E; ++q) d[t + q * T] = v[q];
-  }
-};
 
 // ---------------------------------------------------------------- C1: column FFT + amplitude
 struct BaseFarFieldCols {
@@ -144,71 +108,8 @@ __global__ void __launch_bounds__(ColCfg<N>::T * C)
   for (int q = 0; q < E; ++q) tb[off + (int64_t)(t + q * T) * width] = v[q];
 }
 
-// ---------------------------------------------------------------- R2: row iFFT + scatter + probe partials
-template <int N>
-__global__ void __launch_bounds__(256)
-    k_base_scatter_rows(const float2* __restrict__ tb, const float2* __restrict__ sample, int64_t H, int64_t W,
-                        const float2* __restrict__ probe, int h, const int64_t* __restrict__ pos, int64_t B,
-                        int chunk, int ngroups, float2* __restrict__ gobj, float2* __restrict__ contrib,
-                        float2* __restrict__ gp_part) {
-  using Cf = RowCfg<N>;
-  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
-  __shared__ float2 lds[S * lds_len<N>()];
-  const int seq = threadIdx.x / T;
-  const int t = threadIdx.x % T;
-  const int group = blockIdx.x % ngroups;
-  const int ch = blockIdx.x / ngroups;
-  const int r = group * S + seq;
-  const bool rv = r < h;
-  const int64_t b0 = (int64_t)ch * chunk;
-  const int64_t b1 = b0 + chunk < B ? b0 + chunk : B;
-  float2 pr[E], acc[E];
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    pr[q] = rv ? probe[(int64_t)r * N + t + q * T] : c_zero();
-    acc[q] = c_zero();
-  }
-  for (int64_t b = b0; b < b1; ++b) {
-    float2 v[E];
-    const float2* src = tb + (b * h + r) * N;
-#pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = rv ? src[t + q * T] : c_zero();
-    fft_seq<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
-    const int64_t x0 = pos[2 * b], y0 = pos[2 * b + 1];
-    if (rv && window_ok(x0, y0, h, N, H, W)) {
-      const int64_t obase = (y0 + r) * W + x0;
-#pragma unroll
-      for (int q = 0; q < E; ++q) {
-        const int i = t + q * T;
-        const float2 chi = ((r + i) & 1) ? c_neg(v[q]) : v[q];
-        acc[q] = c_add(acc[q], c_mulc(chi, sample[obase + i]));
-        const float2 val = c_mulc(chi, pr[q]);
-        if (contrib) {
-          contrib[(b * h + r) * N + i] = val;
-        } else {
-          unsafeAtomicAdd(&gobj[obase + i].x, val.x);
-          unsafeAtomicAdd(&gobj[obase + i].y, val.y);
-        }
-      }
-    }
-    __syncthreads();  // next view's first LDS write vs this view's last LDS read
-  }
-  if (rv) {
-    float2* d = gp_part + ((int64_t)ch * h + r) * N;
-#pragma unroll
-    for (int q = 0; q < E; ++q) d[t + q * T] = acc[q];
-  }
-}
-
 // ---------------------------------------------------------------- host launchers
 
-template <int N>
-struct GatherRowsRun {
-  static ptx_status run(const BaseGatherRows& op, int64_t nrows, hipStream_t s) {
-    launch_rows<N, -1>(op, nrows, s);
-    return check_launch("base gather rows", s);
-  }
-};
 template <int N>
 struct FarFieldColsRun {
   static ptx_status run(const BaseFarFieldCols& op, int64_t frames, hipStream_t s) {
@@ -245,121 +146,6 @@ struct StepColsRun {
   }
 };
 
-template <int N>
-struct ColTilesQuery {
-  static ptx_status run(int64_t width, int64_t* tiles) {
-    using Cf = ColCfg<N>;
-    *tiles = width >= Cf::CMAX ? width / Cf::CMAX : width / 8;
-    return PTX_OK;
-  }
-};
-
-struct ScatterPlan {
-  int S;        // rows per work-group
-  int ngroups;  // row groups
-  int chunk;    // views per work-group
-  int nchunks;  // probe partial slices
-};
-
-template <int N>
-struct ScatterPlanQuery {
-  static ptx_status run(int64_t h, int64_t B, ScatterPlan* p) {
-    p->S = RowCfg<N>::S;
-    p->ngroups = (int)((h + p->S - 1) / p->S);
-    // aim for ~1024 work-groups; each probe-partial slice covers `chunk` views
-    int64_t want = (1024 + p->ngroups - 1) / p->ngroups;
-    if (want < 1) want = 1;
-    if (want > B) want = B > 0 ? B : 1;
-    p->chunk = (int)((B + want - 1) / want);
-    if (p->chunk < 1) p->chunk = 1;
-    p->nchunks = (int)((B + p->chunk - 1) / p->chunk);
-    if (p->nchunks < 1) p->nchunks = 1;
-    return PTX_OK;
-  }
-};
-
-template <int N>
-struct ScatterRowsRun {
-  static ptx_status run(const float2* tb, const float2* sample, int64_t H, int64_t W, const float2* probe, int h,
-                        const int64_t* pos, int64_t B, const ScatterPlan& p, float2* gobj, float2* contrib,
-                        float2* gp_part, hipStream_t s) {
-    k_base_scatter_rows<N><<<dim3((unsigned)(p.ngroups * p.nchunks)), dim3(256), 0, s>>>(
-        tb, sample, H, W, probe, h, pos, B, p.chunk, p.ngroups, gobj, contrib, gp_part);
-    return check_launch("base scatter rows", s);
-  }
-};
-
-struct BaseWorkspace {
-  float2* tb;        // [B, h, w]
-  float2* gp_part;   // [nchunks, h, w]
-  float* loss_part;  // [B, col tiles]   (step only)
-  int* lists;        // [ntiles, B]       (deterministic only)
-  int* counts;       // [ntiles]
-  size_t bytes;
-};
-
-BaseWorkspace carve_base(void* base, int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags,
-                         bool step, const ScatterPlan& sp, int64_t col_tiles) {
-  Carver cv{(char*)base, 0};
-  BaseWorkspace ws{};
-  ws.tb = cv.take<float2>((size_t)(B * h * w));
-  ws.gp_part = cv.take<float2>((size_t)(sp.nchunks * h * w));
-  ws.loss_part = step ? cv.take<float>((size_t)(B * col_tiles)) : nullptr;
-  if (flags & PTX_FLAG_DETERMINISTIC) {
-    const TileGrid tg = tile_grid(H, W);
-    ws.lists = cv.take<int>((size_t)(tg.count() * B));
-    ws.counts = cv.take<int>((size_t)tg.count());
-  }
-  ws.bytes = cv.used;
-  return ws;
-}
-
-ptx_status check_shapes(const char* fn, int64_t H, int64_t W, int64_t h, int64_t w, int64_t B) {
-  if (B < 0 || H <= 0 || W <= 0) {
-    set_error("%s: invalid sizes (sample %lld x %lld, batch %lld)", fn, (long long)H, (long long)W, (long long)B);
-    return PTX_ERR_INVALID_ARGUMENT;
-  }
-  if (!size_ok(h) || !size_ok(w)) {
-    set_error("%s: probe %lld x %lld unsupported (sides must be powers of two in [%d, %d])", fn, (long long)h,
-              (long long)w, PTX_MIN_N, PTX_MAX_N);
-    return PTX_ERR_UNSUPPORTED;
-  }
-  if (h > H || w > W) {
-    set_error("%s: probe %lld x %lld larger than sample %lld x %lld", fn, (long long)h, (long long)w, (long long)H,
-              (long long)W);
-    return PTX_ERR_OUT_OF_RANGE;
-  }
-  return PTX_OK;
-}
-
-// shared tail of backward and step: R2 scatter, probe reduction, deterministic gather
-ptx_status base_scatter(const BaseWorkspace& ws, const ScatterPlan& sp, const float2* sample, int64_t H, int64_t W,
-                        const float2* probe, int64_t h, int64_t w, const int64_t* pos, int64_t B, float2* gs,
-                        float2* gp, unsigned flags, hipStream_t s) {
-  const bool det = flags & PTX_FLAG_DETERMINISTIC;
-  const bool acc = flags & PTX_FLAG_ACCUMULATE;
-  ptx_status st = PTX_OK;
-  if (!det && !acc) {
-    fill_zero2(gs, H * W, s);
-    if ((st = check_launch("zero fill", s)) != PTX_OK) return st;
-  }
-  st = dispatch_n<ScatterRowsRun>(w, ws.tb, sample, H, W, probe, (int)h, pos, B, sp, gs, det ? ws.tb : nullptr,
-                                  ws.gp_part, s);
-  if (st != PTX_OK) return st;
-  sum_parts2(ws.gp_part, sp.nchunks, h * w, gp, acc ? 1 : 0, s);
-  if ((st = check_launch("probe partial sum", s)) != PTX_OK) return st;
-  if (det) {
-    const TileGrid tg = tile_grid(H, W);
-    k_tile_lists<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(pos, B, (int)h, (int)w, tg.tiles_x, ws.lists,
-                                                                   ws.counts);
-    if ((st = check_launch("tile lists", s)) != PTX_OK) return st;
-    k_tile_gather<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(ws.tb, (int)h, (int)w, pos, B, ws.lists, ws.counts,
-                                                                    tg.tiles_x, gs, H, W, acc ? 1 : 0);
-    if ((st = check_launch("tile gather", s)) != PTX_OK) return st;
-  }
-  return PTX_OK;
-}
-
 }  // namespace
 }  // namespace ptx
 
@@ -378,7 +164,7 @@ ptx_status ptx_base_forward(const void* sample, int64_t H, int64_t W, const void
   }
   hipStream_t s = (hipStream_t)stream;
   timing_begin(s);
-  BaseGatherRows r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, pos, (float2*)far_field};
+  ScanGatherRows<true> r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, pos, (float2*)far_field};
   st = dispatch_n<GatherRowsRun>(w, r1, B * h, s);
   if (st != PTX_OK) return st;
   BaseFarFieldCols c1{(float2*)far_field, amplitude, w};
@@ -389,7 +175,7 @@ size_t ptx_base_backward_workspace(int64_t H, int64_t W, int64_t h, int64_t w, i
   if (check_shapes("ptx_base_backward_workspace", H, W, h, w, B) != PTX_OK) return 0;
   ScatterPlan sp{};
   dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
-  return carve_base(nullptr, H, W, h, w, B, flags, false, sp, 0).bytes;
+  return carve_scan(nullptr, H, W, h, w, B, flags, false, sp, 0).bytes;
 }
 
 ptx_status ptx_base_backward(const float* grad_output, const void* far_field, const void* sample, int64_t H, int64_t W,
@@ -415,7 +201,7 @@ ptx_status ptx_base_backward(const float* grad_output, const void* far_field, co
   }
   ScatterPlan sp{};
   dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
-  const BaseWorkspace ws = carve_base(workspace, H, W, h, w, B, flags, false, sp, 0);
+  const ScanWorkspace ws = carve_scan(workspace, H, W, h, w, B, flags, false, sp, 0);
   if (!workspace || ws_bytes < ws.bytes) {
     set_error("ptx_base_backward: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
     return PTX_ERR_WORKSPACE;
@@ -424,8 +210,8 @@ ptx_status ptx_base_backward(const float* grad_output, const void* far_field, co
   BaseGammaCols c2{(const float2*)far_field, grad_output, ws.tb, w};
   st = dispatch_n<GammaColsRun>(h, c2, B, s);
   if (st != PTX_OK) return st;
-  return base_scatter(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, pos, B, (float2*)grad_sample,
-                      (float2*)grad_probe, flags, s);
+  return scan_scatter<true>(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, pos, B,
+                            (float2*)grad_sample, (float2*)grad_probe, flags, 1.0f, s);
 }
 
 size_t ptx_base_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
@@ -434,7 +220,7 @@ size_t ptx_base_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64
   dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
   int64_t tiles = 0;
   dispatch_n<ColTilesQuery>(h, w, &tiles);
-  return carve_base(nullptr, H, W, h, w, B, flags, true, sp, tiles).bytes;
+  return carve_scan(nullptr, H, W, h, w, B, flags, true, sp, tiles).bytes;
 }
 
 ptx_status ptx_base_step(const void* sample, int64_t H, int64_t W, const void* probe, int64_t h, int64_t w,
@@ -463,21 +249,21 @@ ptx_status ptx_base_step(const void* sample, int64_t H, int64_t W, const void* p
   dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
   int64_t tiles = 0;
   dispatch_n<ColTilesQuery>(h, w, &tiles);
-  const BaseWorkspace ws = carve_base(workspace, H, W, h, w, B, flags, true, sp, tiles);
+  const ScanWorkspace ws = carve_scan(workspace, H, W, h, w, B, flags, true, sp, tiles);
   if (!workspace || ws_bytes < ws.bytes) {
     set_error("ptx_base_step: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
     return PTX_ERR_WORKSPACE;
   }
   timing_begin(s);
-  BaseGatherRows r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, pos, ws.tb};
+  ScanGatherRows<true> r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, pos, ws.tb};
   st = dispatch_n<GatherRowsRun>(w, r1, B * h, s);
   if (st != PTX_OK) return st;
   st = dispatch_n<StepColsRun>(h, ws.tb, measured, amplitude, ws.loss_part, B, w, 2.0f * loss_scale, s);
   if (st != PTX_OK) return st;
   loss_rows(ws.loss_part, tiles, B, 1.0f / (float)(h * w), loss_per_position, s);
   if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
-  return base_scatter(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, pos, B, (float2*)grad_sample,
-                      (float2*)grad_probe, flags, s);
+  return scan_scatter<true>(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, pos, B,
+                            (float2*)grad_sample, (float2*)grad_probe, flags, 1.0f, s);
 }
 
 }  // extern "C"

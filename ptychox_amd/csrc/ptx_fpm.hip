@@ -1,0 +1,556 @@
+// Fourier ptychographic microscopy (ptychoX_fourier): forward, adjoint and the
+// fused amplitude-loss training step.
+//
+// Reference semantics: python/ptychoX/fpm/ops.py:56-59 (forward), 85-97
+// (backward), 113-124 (wrap-around sub-spectrum indices); CUDA extension
+// python/ptychoX/csrc/fpm_kernel.cu:233-317.
+//
+//   O^ = F O;  sub_b[r,c] = O^[(y_b + r + H/2) mod H, (x_b + c + W/2) mod W]
+//   E_b = ifft2(fftshift(sub_b q)),  A_b = |E_b|
+//   g_b = ifftshift(fft2(G E/max(|E|,eps))) / (h w)
+//   grad_q = sum_b g_b conj(sub_b);  grad_O^[idx_b] += g_b conj(q);  grad_O = H W ifft2(grad_O^)
+//
+// For even h, w:  ifft2(fftshift(X)) = (-1)^(r+c) ifft2(X)  and
+// ifftshift(fft2(Y)) = fft2(Y (-1)^(r+c)), so the shifts are signs in registers
+// and the sub-spectrum is gathered / scattered in its natural order.
+//
+// Passes: object FFT (rows, cols), R1 gather O^ q + row iFFT, C1 column iFFT +
+// scale/sign + amplitude | C2 Gamma sign + column FFT, R2 row FFT + pupil
+// partials + spectrum scatter (atomics or sorted segments), spectrum iFFT (rows,
+// cols).  The step fuses C1 and C2 through the loss.
+#include "common_kernels.hpp"
+#include "passes.hpp"
+#include "ptx_internal.hpp"
+
+namespace ptx {
+namespace {
+
+// ---------------------------------------------------------------- R1: gather sub-spectrum * pupil, row iFFT
+struct FpmGatherRows {
+  const float2* spec;  // [H, W] object spectrum
+  int64_t H, W;
+  const float2* pupil;
+  int h;
+  const int64_t* led;
+  float2* out;  // [B, h, w]
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    const int64_t b = valid ? row / h : 0;
+    const int r = valid ? (int)(row - b * h) : 0;
+    const int64_t sy = pmod(led[2 * b + 1] + r + H / 2, H);
+    const int64_t sx0 = led[2 * b] + W / 2;
+    const float2* srow = spec + sy * W;
+    const float2* prow = pupil + (int64_t)r * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int i = t + q * T;
+      v[q] = valid ? c_mul(srow[pmod(sx0 + i, W)], prow[i]) : c_zero();
+    }
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    if (!valid) return;
+    float2* d = out + row * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) d[t + q * T] = v[q];
+  }
+};
+
+// ---------------------------------------------------------------- C1: column iFFT -> image field, amplitude
+struct FpmImageCols {
+  float2* field;  // in: row-transformed, out: image field E
+  float* amp;
+  int64_t width;
+  float inv_n;  // 1/(h w)
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    const float2* p = field + frame * N * width + col;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = p[(int64_t)(t + q * T) * width];
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    const int64_t off = frame * N * width + col;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int r = t + q * T;
+      const float s = ((r + col) & 1) ? -inv_n : inv_n;
+      const float2 e = c_scale(v[q], s);
+      const int64_t k = off + (int64_t)r * width;
+      field[k] = e;
+      amp[k] = c_abs(e);
+    }
+  }
+};
+
+// ---------------------------------------------------------------- C2: Gamma * sign, column FFT
+struct FpmGammaCols {
+  const float2* field;
+  const float* g;
+  float2* out;
+  int64_t width;
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    const int64_t off = frame * N * width + col;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int r = t + q * T;
+      const int64_t k = off + (int64_t)r * width;
+      const float2 e = field[k];
+      float s = g[k] / fmaxf(c_abs(e), kFltEps);
+      if ((r + col) & 1) s = -s;
+      v[q] = c_scale(e, s);
+    }
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    float2* p = out + frame * N * width + col;
+#pragma unroll
+    for (int q = 0; q < E; ++q) p[(int64_t)(t + q * T) * width] = v[q];
+  }
+};
+
+// ---------------------------------------------------------------- C12: fused column pass of the step
+// column iFFT -> A = |v|/(hw), loss, Gamma*sign = G v/(hw)/max(A, eps) (the two signs cancel) -> column FFT
+template <int N, int C>
+__global__ void __launch_bounds__(ColCfg<N>::T * C)
+    k_fpm_step_cols(float2* __restrict__ tb, const float* __restrict__ measured, float* __restrict__ amp_out,
+                    float* __restrict__ loss_part, int64_t width, int64_t tiles_per_frame, float gscale, float inv_n) {
+  using Cf = ColCfg<N>;
+  constexpr int E = Cf::E, T = Cf::T;
+  __shared__ float2 lds[C * lds_len<N>()];
+  __shared__ float red[16];
+  const int c = threadIdx.x % C;
+  const int t = threadIdx.x / C;
+  const int64_t frame = blockIdx.x / tiles_per_frame;
+  const int64_t tile = blockIdx.x % tiles_per_frame;
+  const int64_t col = tile * C + c;
+  const int64_t off = frame * N * width + col;
+  float2 v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = tb[off + (int64_t)(t + q * T) * width];
+  fft_seq<N, E, +1>(v, t, &lds[c * lds_len<N>()]);
+  float lsum = 0.0f;
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int64_t k = off + (int64_t)(t + q * T) * width;
+    const float a = c_abs(v[q]) * inv_n;
+    const float d = a - measured[k];
+    lsum += d * d;
+    if (amp_out) amp_out[k] = a;
+    v[q] = c_scale(v[q], gscale * d * inv_n / fmaxf(a, kFltEps));
+  }
+  const float tot = block_sum(lsum, red);
+  if (threadIdx.x == 0) loss_part[frame * tiles_per_frame + tile] = tot;
+  fft_seq<N, E, -1>(v, t, &lds[c * lds_len<N>()]);
+#pragma unroll
+  for (int q = 0; q < E; ++q) tb[off + (int64_t)(t + q * T) * width] = v[q];
+}
+
+// ---------------------------------------------------------------- R2: row FFT, pupil partials, spectrum scatter
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_fpm_scatter_rows(const float2* __restrict__ tb, const float2* __restrict__ spec, int64_t H, int64_t W,
+                       const float2* __restrict__ pupil, int h, const int64_t* __restrict__ led, int64_t B, int chunk,
+                       int ngroups, float inv_n, float2* __restrict__ gspec, float2* __restrict__ contrib,
+                       float2* __restrict__ gq_part) {
+  using Cf = RowCfg<N>;
+  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  __shared__ float2 lds[S * lds_len<N>()];
+  const int seq = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int group = blockIdx.x % ngroups;
+  const int ch = blockIdx.x / ngroups;
+  const int r = group * S + seq;
+  const bool rv = r < h;
+  const int64_t b0 = (int64_t)ch * chunk;
+  const int64_t b1 = b0 + chunk < B ? b0 + chunk : B;
+  float2 qc[E], acc[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    qc[q] = rv ? pupil[(int64_t)r * N + t + q * T] : c_zero();
+    acc[q] = c_zero();
+  }
+  for (int64_t b = b0; b < b1; ++b) {
+    float2 v[E];
+    const float2* src = tb + (b * h + r) * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = rv ? src[t + q * T] : c_zero();
+    fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
+    if (rv) {
+      const int64_t sy = pmod(led[2 * b + 1] + r + H / 2, H);
+      const int64_t sx0 = led[2 * b] + W / 2;
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const int i = t + q * T;
+        const float2 g = c_scale(v[q], inv_n);
+        const int64_t k = sy * W + pmod(sx0 + i, W);
+        acc[q] = c_add(acc[q], c_mulc(g, spec[k]));
+        const float2 val = c_mulc(g, qc[q]);
+        if (contrib) {
+          contrib[(b * h + r) * N + i] = val;
+        } else {
+          unsafeAtomicAdd(&gspec[k].x, val.x);
+          unsafeAtomicAdd(&gspec[k].y, val.y);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (rv) {
+    float2* d = gq_part + ((int64_t)ch * h + r) * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) d[t + q * T] = acc[q];
+  }
+}
+
+// columns of [N, width] into `out` (= or +=), used for the last pass of grad_O = HW ifft2(grad_O^)
+struct ColsToOut {
+  const float2* in;
+  float2* out;
+  int64_t width;
+  int accumulate;
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    const float2* p = in + frame * N * width + col;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = p[(int64_t)(t + q * T) * width];
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    float2* p = out + frame * N * width + col;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      float2* d = p + (int64_t)(t + q * T) * width;
+      *d = accumulate ? c_add(*d, v[q]) : v[q];
+    }
+  }
+};
+
+// ---------------------------------------------------------------- host launchers
+
+template <int N>
+struct RowsRun {
+  template <int DIR, class Op>
+  static ptx_status go(const Op& op, int64_t nrows, hipStream_t s, const char* what) {
+    launch_rows<N, DIR>(op, nrows, s);
+    return check_launch(what, s);
+  }
+};
+template <int N>
+struct ObjFftRows {
+  static ptx_status run(const float2* in, float2* out, int64_t nrows, int dir, hipStream_t s) {
+    RowsOutOfPlace op{in, out, 1.0f};
+    return dir < 0 ? RowsRun<N>::template go<-1>(op, nrows, s, "fft rows")
+                   : RowsRun<N>::template go<+1>(op, nrows, s, "ifft rows");
+  }
+};
+template <int N>
+struct ObjFftColsOut {
+  static ptx_status run(const ColsToOut& op, int dir, hipStream_t s) {
+    if (dir < 0)
+      launch_cols<N, -1>(op, 1, op.width, s);
+    else
+      launch_cols<N, +1>(op, 1, op.width, s);
+    return check_launch(dir < 0 ? "fft cols" : "ifft cols", s);
+  }
+};
+template <int N>
+struct FpmGatherRun {
+  static ptx_status run(const FpmGatherRows& op, int64_t nrows, hipStream_t s) {
+    launch_rows<N, +1>(op, nrows, s);
+    return check_launch("fpm gather rows", s);
+  }
+};
+template <int N>
+struct FpmImageColsRun {
+  static ptx_status run(const FpmImageCols& op, int64_t frames, hipStream_t s) {
+    launch_cols<N, +1>(op, frames, op.width, s);
+    return check_launch("fpm image cols", s);
+  }
+};
+template <int N>
+struct FpmGammaColsRun {
+  static ptx_status run(const FpmGammaCols& op, int64_t frames, hipStream_t s) {
+    launch_cols<N, -1>(op, frames, op.width, s);
+    return check_launch("fpm gamma cols", s);
+  }
+};
+template <int N>
+struct FpmStepColsRun {
+  static ptx_status run(float2* tb, const float* measured, float* amp, float* loss_part, int64_t frames, int64_t width,
+                        float gscale, float inv_n, hipStream_t s) {
+    using Cf = ColCfg<N>;
+    if (width >= Cf::CMAX) {
+      constexpr int C = Cf::CMAX;
+      const int64_t tiles = width / C;
+      k_fpm_step_cols<N, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(
+          tb, measured, amp, loss_part, width, tiles, gscale, inv_n);
+    } else {
+      constexpr int C = 8;
+      const int64_t tiles = width / C;
+      k_fpm_step_cols<N, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(
+          tb, measured, amp, loss_part, width, tiles, gscale, inv_n);
+    }
+    return check_launch("fpm step cols", s);
+  }
+};
+template <int N>
+struct ColTiles {
+  static ptx_status run(int64_t width, int64_t* tiles) {
+    *tiles = width >= ColCfg<N>::CMAX ? width / ColCfg<N>::CMAX : width / 8;
+    return PTX_OK;
+  }
+};
+
+struct Plan {
+  int S, ngroups, chunk, nchunks;
+};
+template <int N>
+struct PlanQuery {
+  static ptx_status run(int64_t h, int64_t B, Plan* p) {
+    p->S = RowCfg<N>::S;
+    p->ngroups = (int)((h + p->S - 1) / p->S);
+    int64_t want = (1024 + p->ngroups - 1) / p->ngroups;
+    if (want > B) want = B > 0 ? B : 1;
+    if (want < 1) want = 1;
+    p->chunk = (int)((B + want - 1) / want);
+    if (p->chunk < 1) p->chunk = 1;
+    p->nchunks = (int)((B + p->chunk - 1) / p->chunk);
+    if (p->nchunks < 1) p->nchunks = 1;
+    return PTX_OK;
+  }
+};
+template <int N>
+struct ScatterRun {
+  static ptx_status run(const float2* tb, const float2* spec, int64_t H, int64_t W, const float2* pupil, int h,
+                        const int64_t* led, int64_t B, const Plan& p, float inv_n, float2* gspec, float2* contrib,
+                        float2* gq_part, hipStream_t s) {
+    k_fpm_scatter_rows<N><<<dim3((unsigned)(p.ngroups * p.nchunks)), dim3(256), 0, s>>>(
+        tb, spec, H, W, pupil, h, led, B, p.chunk, p.ngroups, inv_n, gspec, contrib, gq_part);
+    return check_launch("fpm scatter rows", s);
+  }
+};
+
+struct FpmWorkspace {
+  float2* spec;      // [H, W] object spectrum (step only)
+  float2* gspec;     // [H, W] spectrum gradient
+  float2* tb;        // [B, h, w]
+  float2* gq_part;   // [nchunks, h, w]
+  float* loss_part;  // [B, col tiles] (step only)
+  int* lists;        // [ntiles, B] (deterministic)
+  int* counts;
+  size_t bytes;
+};
+
+FpmWorkspace carve(void* base, int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags, bool step,
+                   const Plan& p, int64_t tiles) {
+  Carver cv{(char*)base, 0};
+  FpmWorkspace ws{};
+  ws.spec = step ? cv.take<float2>((size_t)(H * W)) : nullptr;
+  ws.gspec = cv.take<float2>((size_t)(H * W));
+  ws.tb = cv.take<float2>((size_t)(B * h * w));
+  ws.gq_part = cv.take<float2>((size_t)(p.nchunks * h * w));
+  ws.loss_part = step ? cv.take<float>((size_t)(B * tiles)) : nullptr;
+  if (flags & PTX_FLAG_DETERMINISTIC) {
+    const TileGrid tg = tile_grid(H, W);
+    ws.lists = cv.take<int>((size_t)(tg.count() * B));
+    ws.counts = cv.take<int>((size_t)tg.count());
+  }
+  ws.bytes = cv.used;
+  return ws;
+}
+
+ptx_status check(const char* fn, int64_t H, int64_t W, int64_t h, int64_t w, int64_t B) {
+  if (B < 0) {
+    set_error("%s: negative batch", fn);
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  if (!size_ok(H) || !size_ok(W) || !size_ok(h) || !size_ok(w)) {
+    set_error("%s: object %lld x %lld / pupil %lld x %lld unsupported (sides must be powers of two in [%d, %d])", fn,
+              (long long)H, (long long)W, (long long)h, (long long)w, PTX_MIN_N, PTX_MAX_N);
+    return PTX_ERR_UNSUPPORTED;
+  }
+  if (h > H || w > W) {
+    set_error("%s: pupil %lld x %lld larger than the object spectrum %lld x %lld", fn, (long long)h, (long long)w,
+              (long long)H, (long long)W);
+    return PTX_ERR_OUT_OF_RANGE;
+  }
+  return PTX_OK;
+}
+
+// object spectrum: rows (out of place) then columns into `spec`
+ptx_status spectrum_of(const float2* obj, float2* spec, int64_t H, int64_t W, hipStream_t s) {
+  ptx_status st = dispatch_n<ObjFftRows>(W, obj, spec, H, -1, s);
+  if (st != PTX_OK) return st;
+  ColsToOut c{spec, spec, W, 0};
+  return dispatch_n<ObjFftColsOut>(H, c, -1, s);
+}
+
+// R2 + pupil reduction + spectrum scatter + grad_O = HW ifft2(grad_O^)
+ptx_status fpm_tail(const FpmWorkspace& ws, const Plan& p, const float2* spec, int64_t H, int64_t W,
+                    const float2* pupil, int64_t h, int64_t w, const int64_t* led, int64_t B, float2* gobj,
+                    float2* gpupil, unsigned flags, hipStream_t s) {
+  const bool det = flags & PTX_FLAG_DETERMINISTIC;
+  const bool acc = flags & PTX_FLAG_ACCUMULATE;
+  ptx_status st = PTX_OK;
+  if (!det) {
+    fill_zero2(ws.gspec, H * W, s);
+    if ((st = check_launch("zero fill", s)) != PTX_OK) return st;
+  }
+  const float inv_n = 1.0f / (float)(h * w);
+  st = dispatch_n<ScatterRun>(w, ws.tb, spec, H, W, pupil, (int)h, led, B, p, inv_n, ws.gspec, det ? ws.tb : nullptr,
+                              ws.gq_part, s);
+  if (st != PTX_OK) return st;
+  sum_parts2(ws.gq_part, p.nchunks, h * w, gpupil, acc ? 1 : 0, s);
+  if ((st = check_launch("pupil partial sum", s)) != PTX_OK) return st;
+  if (det) {
+    const TileGrid tg = tile_grid(H, W);
+    k_tile_lists<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(led, B, (int)h, (int)w, H, W, H / 2, W / 2,
+                                                                   tg.tiles_x, ws.lists, ws.counts);
+    if ((st = check_launch("tile lists", s)) != PTX_OK) return st;
+    k_tile_gather<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(ws.tb, (int)h, (int)w, led, B, H / 2, W / 2,
+                                                                    ws.lists, ws.counts, tg.tiles_x, ws.gspec, H, W, 0);
+    if ((st = check_launch("tile gather", s)) != PTX_OK) return st;
+  }
+  st = dispatch_n<ObjFftRows>(W, ws.gspec, ws.gspec, H, +1, s);
+  if (st != PTX_OK) return st;
+  ColsToOut c{ws.gspec, gobj, W, acc ? 1 : 0};
+  return dispatch_n<ObjFftColsOut>(H, c, +1, s);
+}
+
+}  // namespace
+}  // namespace ptx
+
+using namespace ptx;
+
+extern "C" {
+
+ptx_status ptx_fpm_forward(const void* object_field, int64_t H, int64_t W, const void* pupil, int64_t h, int64_t w,
+                           const int64_t* led, int64_t B, void* object_fft, void* image_field, float* amplitude,
+                           void* stream) {
+  ptx_status st = check("ptx_fpm_forward", H, W, h, w, B);
+  if (st != PTX_OK) return st;
+  if (!object_field || !pupil || !object_fft || (B > 0 && (!led || !image_field || !amplitude))) {
+    set_error("ptx_fpm_forward: null pointer argument");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  timing_begin(s);
+  st = spectrum_of((const float2*)object_field, (float2*)object_fft, H, W, s);
+  if (st != PTX_OK || B == 0) return st;
+  FpmGatherRows r1{(const float2*)object_fft, H, W, (const float2*)pupil, (int)h, led, (float2*)image_field};
+  st = dispatch_n<FpmGatherRun>(w, r1, B * h, s);
+  if (st != PTX_OK) return st;
+  FpmImageCols c1{(float2*)image_field, amplitude, w, 1.0f / (float)(h * w)};
+  return dispatch_n<FpmImageColsRun>(h, c1, B, s);
+}
+
+size_t ptx_fpm_backward_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
+  if (check("ptx_fpm_backward_workspace", H, W, h, w, B) != PTX_OK) return 0;
+  Plan p{};
+  dispatch_n<PlanQuery>(w, h, B, &p);
+  return carve(nullptr, H, W, h, w, B, flags, false, p, 0).bytes;
+}
+
+ptx_status ptx_fpm_backward(const float* grad_output, const void* image_field, const void* object_fft, int64_t H,
+                            int64_t W, const void* pupil, int64_t h, int64_t w, const int64_t* led, int64_t B,
+                            void* grad_object, void* grad_pupil, void* workspace, size_t ws_bytes, unsigned flags,
+                            void* stream) {
+  ptx_status st = check("ptx_fpm_backward", H, W, h, w, B);
+  if (st != PTX_OK) return st;
+  hipStream_t s = (hipStream_t)stream;
+  if (!grad_object || !grad_pupil) {
+    set_error("ptx_fpm_backward: null gradient output");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  timing_begin(s);
+  if (B == 0) {
+    if (!(flags & PTX_FLAG_ACCUMULATE)) {
+      fill_zero2((float2*)grad_object, H * W, s);
+      fill_zero2((float2*)grad_pupil, h * w, s);
+    }
+    return check_launch("zero fill", s);
+  }
+  if (!grad_output || !image_field || !object_fft || !pupil || !led) {
+    set_error("ptx_fpm_backward: null pointer argument");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  Plan p{};
+  dispatch_n<PlanQuery>(w, h, B, &p);
+  const FpmWorkspace ws = carve(workspace, H, W, h, w, B, flags, false, p, 0);
+  if (!workspace || ws_bytes < ws.bytes) {
+    set_error("ptx_fpm_backward: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
+    return PTX_ERR_WORKSPACE;
+  }
+  FpmGammaCols c2{(const float2*)image_field, grad_output, ws.tb, w};
+  st = dispatch_n<FpmGammaColsRun>(h, c2, B, s);
+  if (st != PTX_OK) return st;
+  return fpm_tail(ws, p, (const float2*)object_fft, H, W, (const float2*)pupil, h, w, led, B, (float2*)grad_object,
+                  (float2*)grad_pupil, flags, s);
+}
+
+size_t ptx_fpm_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
+  if (check("ptx_fpm_step_workspace", H, W, h, w, B) != PTX_OK) return 0;
+  Plan p{};
+  dispatch_n<PlanQuery>(w, h, B, &p);
+  int64_t tiles = 0;
+  dispatch_n<ColTiles>(h, w, &tiles);
+  return carve(nullptr, H, W, h, w, B, flags, true, p, tiles).bytes;
+}
+
+ptx_status ptx_fpm_step(const void* object_field, int64_t H, int64_t W, const void* pupil, int64_t h, int64_t w,
+                        const int64_t* led, int64_t B, const float* measured, float loss_scale,
+                        float* loss_per_position, float* amplitude, void* grad_object, void* grad_pupil,
+                        void* workspace, size_t ws_bytes, unsigned flags, void* stream) {
+  ptx_status st = check("ptx_fpm_step", H, W, h, w, B);
+  if (st != PTX_OK) return st;
+  hipStream_t s = (hipStream_t)stream;
+  if (!grad_object || !grad_pupil) {
+    set_error("ptx_fpm_step: null gradient output");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  timing_begin(s);
+  if (B == 0) {
+    if (!(flags & PTX_FLAG_ACCUMULATE)) {
+      fill_zero2((float2*)grad_object, H * W, s);
+      fill_zero2((float2*)grad_pupil, h * w, s);
+    }
+    return check_launch("zero fill", s);
+  }
+  if (!object_field || !pupil || !led || !measured || !loss_per_position) {
+    set_error("ptx_fpm_step: null pointer argument");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  Plan p{};
+  dispatch_n<PlanQuery>(w, h, B, &p);
+  int64_t tiles = 0;
+  dispatch_n<ColTiles>(h, w, &tiles);
+  const FpmWorkspace ws = carve(workspace, H, W, h, w, B, flags, true, p, tiles);
+  if (!workspace || ws_bytes < ws.bytes) {
+    set_error("ptx_fpm_step: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
+    return PTX_ERR_WORKSPACE;
+  }
+  st = spectrum_of((const float2*)object_field, ws.spec, H, W, s);
+  if (st != PTX_OK) return st;
+  FpmGatherRows r1{ws.spec, H, W, (const float2*)pupil, (int)h, led, ws.tb};
+  st = dispatch_n<FpmGatherRun>(w, r1, B * h, s);
+  if (st != PTX_OK) return st;
+  const float inv_n = 1.0f / (float)(h * w);
+  st = dispatch_n<FpmStepColsRun>(h, ws.tb, measured, amplitude, ws.loss_part, B, w, 2.0f * loss_scale, inv_n, s);
+  if (st != PTX_OK) return st;
+  loss_rows(ws.loss_part, tiles, B, inv_n, loss_per_position, s);
+  if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
+  return fpm_tail(ws, p, ws.spec, H, W, (const float2*)pupil, h, w, led, B, (float2*)grad_object,
+                  (float2*)grad_pupil, flags, s);
+}
+
+}  // extern "C"

@@ -9,7 +9,7 @@
 #include <cstdio>
 #include <string>
 
-#include "../../include/ptx.h"
+#include "ptx.h"
 #include "fft_core.hpp"
 
 namespace ptx {

@@ -1,0 +1,325 @@
+// Near-field ptychography (ptychoX_nearfield): forward, adjoint and the fused
+// amplitude-loss training step.
+//
+// The reference implements near-field only in MATLAB, as a full-frame
+// propagation (matlab/PtychoX_updated/scripts/forward/fwd_adPtychoX_nearf.cu:58-114:
+// FFT, pixel-wise transfer multiply, inverse FFT, detector magnitude).  This
+// operator applies that angular-spectrum propagation to each probe-illuminated
+// scan window, as the conventional operator does with its far-field FFT:
+//   Psi_b = ifft2(T fft2(P_b p)),   A_b = |Psi_b|          (T has DC at [0, 0])
+//   chi_b = ifft2(conj(T) fft2(G Psi/max(|Psi|, eps)))     (adjoint of F^-1 D_T F)
+//   grad_sample[win_b] += chi_b conj(p),  grad_probe += sum_b chi_b conj(P_b)
+//
+// Passes: R1 gather * probe + row FFT | C_T column FFT, * T, column iFFT |
+// R3 row iFFT / N -> field, amplitude.  Backward: gamma + row FFT | C_T with
+// conj(T) | row iFFT / N + scatter.  The step joins R3 and the gamma row pass
+// through the loss (five passes in all).
+#include "scan_common.hpp"
+
+namespace ptx {
+namespace {
+
+// column FFT, * T (or conj T), column iFFT -- in place
+template <int N, int C, bool CONJ>
+__global__ void __launch_bounds__(ColCfg<N>::T * C)
+    k_transfer_cols(float2* __restrict__ tb, const float2* __restrict__ transfer, int64_t width,
+                    int64_t tiles_per_frame) {
+  using Cf = ColCfg<N>;
+  constexpr int E = Cf::E, T = Cf::T;
+  __shared__ float2 lds[C * lds_len<N>()];
+  const int c = threadIdx.x % C;
+  const int t = threadIdx.x / C;
+  const int64_t frame = blockIdx.x / tiles_per_frame;
+  const int64_t col = (blockIdx.x % tiles_per_frame) * C + c;
+  float2* p = tb + frame * N * width + col;
+  float2 v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = p[(int64_t)(t + q * T) * width];
+  fft_seq<N, E, -1>(v, t, &lds[c * lds_len<N>()]);
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const float2 tr = transfer[(int64_t)(t + q * T) * width + col];
+    v[q] = CONJ ? c_mulc(v[q], tr) : c_mul(v[q], tr);
+  }
+  __syncthreads();  // second transform reuses the LDS region
+  fft_seq<N, E, +1>(v, t, &lds[c * lds_len<N>()]);
+#pragma unroll
+  for (int q = 0; q < E; ++q) p[(int64_t)(t + q * T) * width] = v[q];
+}
+
+// row iFFT / N -> field and amplitude
+struct NfFieldRows {
+  const float2* in;
+  float2* field;
+  float* amp;
+  float inv_n;
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    const float2* p = in + row * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = valid ? p[t + q * T] : c_zero();
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    if (!valid) return;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int64_t k = row * N + t + q * T;
+      const float2 e = c_scale(v[q], inv_n);
+      field[k] = e;
+      amp[k] = c_abs(e);
+    }
+  }
+};
+
+// Gamma = G Psi / max(|Psi|, eps), row FFT
+struct NfGammaRows {
+  const float2* field;
+  const float* g;
+  float2* out;
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int64_t k = row * N + t + q * T;
+      if (valid) {
+        const float2 e = field[k];
+        v[q] = c_scale(e, g[k] / fmaxf(c_abs(e), kFltEps));
+      } else {
+        v[q] = c_zero();
+      }
+    }
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    if (!valid) return;
+    float2* p = out + row * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) p[t + q * T] = v[q];
+  }
+};
+
+// step: row iFFT / N -> A, loss partial per row, Gamma, row FFT (in place)
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_nf_step_rows(float2* __restrict__ tb, const float* __restrict__ measured, float* __restrict__ amp_out,
+                   float* __restrict__ loss_part, int64_t nrows, float gscale, float inv_n) {
+  using Cf = RowCfg<N>;
+  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  __shared__ float2 lds[S * lds_len<N>()];
+  __shared__ float red[256];
+  const int seq = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int64_t row = (int64_t)blockIdx.x * S + seq;
+  const bool valid = row < nrows;
+  float2* p = tb + row * N;
+  float2 v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = valid ? p[t + q * T] : c_zero();
+  fft_seq<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
+  float lsum = 0.0f;
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int64_t k = row * N + t + q * T;
+    const float2 e = c_scale(v[q], inv_n);
+    const float a = c_abs(e);
+    const float d = valid ? a - measured[k] : 0.0f;
+    lsum += d * d;
+    if (valid && amp_out) amp_out[k] = a;
+    v[q] = c_scale(e, gscale * d / fmaxf(a, kFltEps));
+  }
+  red[threadIdx.x] = lsum;
+  __syncthreads();  // also orders the LDS reuse of the second transform
+  if (valid && t == 0) {
+    float sum = 0.0f;
+    for (int k = 0; k < T; ++k) sum += red[seq * T + k];  // fixed order
+    loss_part[row] = sum;
+  }
+  fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) p[t + q * T] = v[q];
+  }
+}
+
+template <int N>
+struct TransferColsRun {
+  static ptx_status run(float2* tb, const float2* transfer, int64_t frames, int64_t width, bool conj, hipStream_t s) {
+    using Cf = ColCfg<N>;
+    const int C = width >= Cf::CMAX ? Cf::CMAX : 8;
+    const int64_t tiles = width / C;
+    const dim3 grid((unsigned)(frames * tiles));
+    if (C == Cf::CMAX) {
+      if (conj)
+        k_transfer_cols<N, Cf::CMAX, true><<<grid, dim3(Cf::T * Cf::CMAX), 0, s>>>(tb, transfer, width, tiles);
+      else
+        k_transfer_cols<N, Cf::CMAX, false><<<grid, dim3(Cf::T * Cf::CMAX), 0, s>>>(tb, transfer, width, tiles);
+    } else {
+      if (conj)
+        k_transfer_cols<N, 8, true><<<grid, dim3(Cf::T * 8), 0, s>>>(tb, transfer, width, tiles);
+      else
+        k_transfer_cols<N, 8, false><<<grid, dim3(Cf::T * 8), 0, s>>>(tb, transfer, width, tiles);
+    }
+    return check_launch(conj ? "nf transfer* cols" : "nf transfer cols", s);
+  }
+};
+template <int N>
+struct FieldRowsRun {
+  static ptx_status run(const NfFieldRows& op, int64_t nrows, hipStream_t s) {
+    launch_rows<N, +1>(op, nrows, s);
+    return check_launch("nf field rows", s);
+  }
+};
+template <int N>
+struct GammaRowsRun {
+  static ptx_status run(const NfGammaRows& op, int64_t nrows, hipStream_t s) {
+    launch_rows<N, -1>(op, nrows, s);
+    return check_launch("nf gamma rows", s);
+  }
+};
+template <int N>
+struct StepRowsRun {
+  static ptx_status run(float2* tb, const float* measured, float* amp, float* loss_part, int64_t nrows, float gscale,
+                        float inv_n, hipStream_t s) {
+    const int64_t blocks = (nrows + RowCfg<N>::S - 1) / RowCfg<N>::S;
+    k_nf_step_rows<N><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(tb, measured, amp, loss_part, nrows, gscale, inv_n);
+    return check_launch("nf step rows", s);
+  }
+};
+
+ptx_status nf_check(const char* fn, int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, const void* transfer) {
+  ptx_status st = check_shapes(fn, H, W, h, w, B);
+  if (st != PTX_OK) return st;
+  if (B > 0 && !transfer) {
+    set_error("%s: null transfer function", fn);
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  return PTX_OK;
+}
+
+}  // namespace
+}  // namespace ptx
+
+using namespace ptx;
+
+extern "C" {
+
+ptx_status ptx_nearfield_forward(const void* sample, int64_t H, int64_t W, const void* probe, int64_t h, int64_t w,
+                                 const void* transfer, const int64_t* pos, int64_t B, void* field, float* amplitude,
+                                 void* stream) {
+  ptx_status st = nf_check("ptx_nearfield_forward", H, W, h, w, B, transfer);
+  if (st != PTX_OK || B == 0) return st;
+  if (!sample || !probe || !pos || !field || !amplitude) {
+    set_error("ptx_nearfield_forward: null pointer argument");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  timing_begin(s);
+  ScanGatherRows<false> r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, pos, (float2*)field};
+  if ((st = dispatch_n<GatherRowsRun>(w, r1, B * h, s)) != PTX_OK) return st;
+  if ((st = dispatch_n<TransferColsRun>(h, (float2*)field, (const float2*)transfer, B, w, false, s)) != PTX_OK)
+    return st;
+  NfFieldRows r3{(const float2*)field, (float2*)field, amplitude, 1.0f / (float)(h * w)};
+  return dispatch_n<FieldRowsRun>(w, r3, B * h, s);
+}
+
+size_t ptx_nearfield_backward_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
+  if (check_shapes("ptx_nearfield_backward_workspace", H, W, h, w, B) != PTX_OK) return 0;
+  ScatterPlan sp{};
+  dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
+  return carve_scan(nullptr, H, W, h, w, B, flags, false, sp, 0).bytes;
+}
+
+ptx_status ptx_nearfield_backward(const float* grad_output, const void* field, const void* sample, int64_t H,
+                                  int64_t W, const void* probe, int64_t h, int64_t w, const void* transfer,
+                                  const int64_t* pos, int64_t B, void* grad_sample, void* grad_probe, void* workspace,
+                                  size_t ws_bytes, unsigned flags, void* stream) {
+  ptx_status st = nf_check("ptx_nearfield_backward", H, W, h, w, B, transfer);
+  if (st != PTX_OK) return st;
+  hipStream_t s = (hipStream_t)stream;
+  if (!grad_sample || !grad_probe) {
+    set_error("ptx_nearfield_backward: null gradient output");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  timing_begin(s);
+  if (B == 0) {
+    if (!(flags & PTX_FLAG_ACCUMULATE)) {
+      fill_zero2((float2*)grad_sample, H * W, s);
+      fill_zero2((float2*)grad_probe, h * w, s);
+    }
+    return check_launch("zero fill", s);
+  }
+  if (!grad_output || !field || !sample || !probe || !pos) {
+    set_error("ptx_nearfield_backward: null pointer argument");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  ScatterPlan sp{};
+  dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
+  const ScanWorkspace ws = carve_scan(workspace, H, W, h, w, B, flags, false, sp, 0);
+  if (!workspace || ws_bytes < ws.bytes) {
+    set_error("ptx_nearfield_backward: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
+    return PTX_ERR_WORKSPACE;
+  }
+  NfGammaRows r1{(const float2*)field, grad_output, ws.tb};
+  if ((st = dispatch_n<GammaRowsRun>(w, r1, B * h, s)) != PTX_OK) return st;
+  if ((st = dispatch_n<TransferColsRun>(h, ws.tb, (const float2*)transfer, B, w, true, s)) != PTX_OK) return st;
+  return scan_scatter<false>(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, pos, B,
+                             (float2*)grad_sample, (float2*)grad_probe, flags, 1.0f / (float)(h * w), s);
+}
+
+size_t ptx_nearfield_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
+  if (check_shapes("ptx_nearfield_step_workspace", H, W, h, w, B) != PTX_OK) return 0;
+  ScatterPlan sp{};
+  dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
+  return carve_scan(nullptr, H, W, h, w, B, flags, true, sp, h).bytes;
+}
+
+ptx_status ptx_nearfield_step(const void* sample, int64_t H, int64_t W, const void* probe, int64_t h, int64_t w,
+                              const void* transfer, const int64_t* pos, int64_t B, const float* measured,
+                              float loss_scale, float* loss_per_position, float* amplitude, void* grad_sample,
+                              void* grad_probe, void* workspace, size_t ws_bytes, unsigned flags, void* stream) {
+  ptx_status st = nf_check("ptx_nearfield_step", H, W, h, w, B, transfer);
+  if (st != PTX_OK) return st;
+  hipStream_t s = (hipStream_t)stream;
+  if (!grad_sample || !grad_probe) {
+    set_error("ptx_nearfield_step: null gradient output");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  timing_begin(s);
+  if (B == 0) {
+    if (!(flags & PTX_FLAG_ACCUMULATE)) {
+      fill_zero2((float2*)grad_sample, H * W, s);
+      fill_zero2((float2*)grad_probe, h * w, s);
+    }
+    return check_launch("zero fill", s);
+  }
+  if (!sample || !probe || !pos || !measured || !loss_per_position) {
+    set_error("ptx_nearfield_step: null pointer argument");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  ScatterPlan sp{};
+  dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
+  const ScanWorkspace ws = carve_scan(workspace, H, W, h, w, B, flags, true, sp, h);
+  if (!workspace || ws_bytes < ws.bytes) {
+    set_error("ptx_nearfield_step: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
+    return PTX_ERR_WORKSPACE;
+  }
+  const float inv_n = 1.0f / (float)(h * w);
+  ScanGatherRows<false> r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, pos, ws.tb};
+  if ((st = dispatch_n<GatherRowsRun>(w, r1, B * h, s)) != PTX_OK) return st;
+  if ((st = dispatch_n<TransferColsRun>(h, ws.tb, (const float2*)transfer, B, w, false, s)) != PTX_OK) return st;
+  if ((st = dispatch_n<StepRowsRun>(w, ws.tb, measured, amplitude, ws.loss_part, B * h, 2.0f * loss_scale, inv_n,
+                                    s)) != PTX_OK)
+    return st;
+  loss_rows(ws.loss_part, h, B, inv_n, loss_per_position, s);
+  if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
+  if ((st = dispatch_n<TransferColsRun>(h, ws.tb, (const float2*)transfer, B, w, true, s)) != PTX_OK) return st;
+  return scan_scatter<false>(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, pos, B,
+                             (float2*)grad_sample, (float2*)grad_probe, flags, inv_n, s);
+}
+
+}  // extern "C"

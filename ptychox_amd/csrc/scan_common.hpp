@@ -1,0 +1,235 @@
+// Pieces shared by the scan-window operators (conventional and near-field):
+// the gather of sample window * probe into row transforms, the row pass that
+// scatters the object gradient and accumulates the probe gradient over a chunk
+// of views in registers, the launch plan and the workspace layout.
+#pragma once
+
+#include <type_traits>
+
+#include "common_kernels.hpp"
+#include "passes.hpp"
+#include "ptx_internal.hpp"
+
+namespace ptx {
+namespace {
+
+// ---------------------------------------------------------------- gather sample window * probe (* sign), row FFT
+template <bool SIGN>
+struct ScanGatherRows {
+  const float2* sample;
+  int64_t H, W;
+  const float2* probe;
+  int h;
+  const int64_t* pos;
+  float2* out;  // [B, h, w]
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    const int64_t b = valid ? row / h : 0;
+    const int r = valid ? (int)(row - b * h) : 0;
+    const int64_t x0 = pos[2 * b], y0 = pos[2 * b + 1];
+    const bool ok = valid && window_ok(x0, y0, h, N, H, W);
+    const float2* srow = sample + (y0 + r) * W + x0;
+    const float2* prow = probe + (int64_t)r * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int i = t + q * T;
+      float2 e = ok ? c_mul(srow[i], prow[i]) : c_zero();
+      v[q] = (SIGN && ((r + i) & 1)) ? c_neg(e) : e;
+    }
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    if (!valid) return;
+    float2* d = out + row * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) d[t + q * T] = v[q];
+  }
+};
+
+// ---------------------------------------------------------------- row iFFT, (sign | scale), object scatter, probe partials
+template <int N, bool SIGN>
+__global__ void __launch_bounds__(256)
+    k_scan_scatter_rows(const float2* __restrict__ tb, const float2* __restrict__ sample, int64_t H, int64_t W,
+                        const float2* __restrict__ probe, int h, const int64_t* __restrict__ pos, int64_t B,
+                        int chunk, int ngroups, float2* __restrict__ gobj, float2* __restrict__ contrib,
+                        float2* __restrict__ gp_part, float scale) {
+  using Cf = RowCfg<N>;
+  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  __shared__ float2 lds[S * lds_len<N>()];
+  const int seq = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int group = blockIdx.x % ngroups;
+  const int ch = blockIdx.x / ngroups;
+  const int r = group * S + seq;
+  const bool rv = r < h;
+  const int64_t b0 = (int64_t)ch * chunk;
+  const int64_t b1 = b0 + chunk < B ? b0 + chunk : B;
+  float2 pr[E], acc[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    pr[q] = rv ? probe[(int64_t)r * N + t + q * T] : c_zero();
+    acc[q] = c_zero();
+  }
+  for (int64_t b = b0; b < b1; ++b) {
+    float2 v[E];
+    const float2* src = tb + (b * h + r) * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = rv ? src[t + q * T] : c_zero();
+    fft_seq<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
+    const int64_t x0 = pos[2 * b], y0 = pos[2 * b + 1];
+    if (rv && window_ok(x0, y0, h, N, H, W)) {
+      const int64_t obase = (y0 + r) * W + x0;
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const int i = t + q * T;
+        const float2 chi = SIGN ? (((r + i) & 1) ? c_neg(v[q]) : v[q]) : c_scale(v[q], scale);
+        acc[q] = c_add(acc[q], c_mulc(chi, sample[obase + i]));
+        const float2 val = c_mulc(chi, pr[q]);
+        if (contrib) {
+          contrib[(b * h + r) * N + i] = val;
+        } else {
+          unsafeAtomicAdd(&gobj[obase + i].x, val.x);
+          unsafeAtomicAdd(&gobj[obase + i].y, val.y);
+        }
+      }
+    }
+    __syncthreads();  // next view's first LDS write vs this view's last LDS read
+  }
+  if (rv) {
+    float2* d = gp_part + ((int64_t)ch * h + r) * N;
+#pragma unroll
+    for (int q = 0; q < E; ++q) d[t + q * T] = acc[q];
+  }
+}
+
+// ---------------------------------------------------------------- host side
+
+template <int N>
+struct GatherRowsRun {
+  template <bool SIGN>
+  static ptx_status run(const ScanGatherRows<SIGN>& op, int64_t nrows, hipStream_t s) {
+    launch_rows<N, -1>(op, nrows, s);
+    return check_launch(SIGN ? "base gather rows" : "scan gather rows", s);
+  }
+};
+template <int N>
+struct ColTilesQuery {
+  static ptx_status run(int64_t width, int64_t* tiles) {
+    using Cf = ColCfg<N>;
+    *tiles = width >= Cf::CMAX ? width / Cf::CMAX : width / 8;
+    return PTX_OK;
+  }
+};
+
+struct ScatterPlan {
+  int S;        // rows per work-group
+  int ngroups;  // row groups
+  int chunk;    // views per work-group
+  int nchunks;  // probe partial slices
+};
+
+template <int N>
+struct ScatterPlanQuery {
+  static ptx_status run(int64_t h, int64_t B, ScatterPlan* p) {
+    p->S = RowCfg<N>::S;
+    p->ngroups = (int)((h + p->S - 1) / p->S);
+    // aim for ~1024 work-groups; each probe-partial slice covers `chunk` views
+    int64_t want = (1024 + p->ngroups - 1) / p->ngroups;
+    if (want < 1) want = 1;
+    if (want > B) want = B > 0 ? B : 1;
+    p->chunk = (int)((B + want - 1) / want);
+    if (p->chunk < 1) p->chunk = 1;
+    p->nchunks = (int)((B + p->chunk - 1) / p->chunk);
+    if (p->nchunks < 1) p->nchunks = 1;
+    return PTX_OK;
+  }
+};
+
+template <int N>
+struct ScatterRowsRun {
+  template <bool SIGN>
+  static ptx_status run(const float2* tb, const float2* sample, int64_t H, int64_t W, const float2* probe, int h,
+                        const int64_t* pos, int64_t B, const ScatterPlan& p, float2* gobj, float2* contrib,
+                        float2* gp_part, float scale, hipStream_t s, std::integral_constant<bool, SIGN>) {
+    k_scan_scatter_rows<N, SIGN><<<dim3((unsigned)(p.ngroups * p.nchunks)), dim3(256), 0, s>>>(
+        tb, sample, H, W, probe, h, pos, B, p.chunk, p.ngroups, gobj, contrib, gp_part, scale);
+    return check_launch(SIGN ? "base scatter rows" : "scan scatter rows", s);
+  }
+};
+
+struct ScanWorkspace {
+  float2* tb;        // [B, h, w]
+  float2* gp_part;   // [nchunks, h, w]
+  float* loss_part;  // [B, col tiles]   (step only)
+  int* lists;        // [ntiles, B]       (deterministic only)
+  int* counts;       // [ntiles]
+  size_t bytes;
+};
+
+ScanWorkspace carve_scan(void* base, int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags,
+                         bool step, const ScatterPlan& sp, int64_t col_tiles) {
+  Carver cv{(char*)base, 0};
+  ScanWorkspace ws{};
+  ws.tb = cv.take<float2>((size_t)(B * h * w));
+  ws.gp_part = cv.take<float2>((size_t)(sp.nchunks * h * w));
+  ws.loss_part = step ? cv.take<float>((size_t)(B * col_tiles)) : nullptr;
+  if (flags & PTX_FLAG_DETERMINISTIC) {
+    const TileGrid tg = tile_grid(H, W);
+    ws.lists = cv.take<int>((size_t)(tg.count() * B));
+    ws.counts = cv.take<int>((size_t)tg.count());
+  }
+  ws.bytes = cv.used;
+  return ws;
+}
+
+ptx_status check_shapes(const char* fn, int64_t H, int64_t W, int64_t h, int64_t w, int64_t B) {
+  if (B < 0 || H <= 0 || W <= 0) {
+    set_error("%s: invalid sizes (sample %lld x %lld, batch %lld)", fn, (long long)H, (long long)W, (long long)B);
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  if (!size_ok(h) || !size_ok(w)) {
+    set_error("%s: probe %lld x %lld unsupported (sides must be powers of two in [%d, %d])", fn, (long long)h,
+              (long long)w, PTX_MIN_N, PTX_MAX_N);
+    return PTX_ERR_UNSUPPORTED;
+  }
+  if (h > H || w > W) {
+    set_error("%s: probe %lld x %lld larger than sample %lld x %lld", fn, (long long)h, (long long)w, (long long)H,
+              (long long)W);
+    return PTX_ERR_OUT_OF_RANGE;
+  }
+  return PTX_OK;
+}
+
+// shared tail of backward and step: R2 scatter, probe reduction, deterministic gather
+template <bool SIGN>
+ptx_status scan_scatter(const ScanWorkspace& ws, const ScatterPlan& sp, const float2* sample, int64_t H, int64_t W,
+                        const float2* probe, int64_t h, int64_t w, const int64_t* pos, int64_t B, float2* gs,
+                        float2* gp, unsigned flags, float scale, hipStream_t s) {
+  const bool det = flags & PTX_FLAG_DETERMINISTIC;
+  const bool acc = flags & PTX_FLAG_ACCUMULATE;
+  ptx_status st = PTX_OK;
+  if (!det && !acc) {
+    fill_zero2(gs, H * W, s);
+    if ((st = check_launch("zero fill", s)) != PTX_OK) return st;
+  }
+  st = dispatch_n<ScatterRowsRun>(w, ws.tb, sample, H, W, probe, (int)h, pos, B, sp, gs, det ? ws.tb : nullptr,
+                                  ws.gp_part, scale, s, std::integral_constant<bool, SIGN>{});
+  if (st != PTX_OK) return st;
+  sum_parts2(ws.gp_part, sp.nchunks, h * w, gp, acc ? 1 : 0, s);
+  if ((st = check_launch("probe partial sum", s)) != PTX_OK) return st;
+  if (det) {
+    const TileGrid tg = tile_grid(H, W);
+    k_tile_lists<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(pos, B, (int)h, (int)w, H, W, 0, 0, tg.tiles_x,
+                                                                   ws.lists, ws.counts);
+    if ((st = check_launch("tile lists", s)) != PTX_OK) return st;
+    k_tile_gather<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(ws.tb, (int)h, (int)w, pos, B, 0, 0, ws.lists,
+                                                                    ws.counts, tg.tiles_x, gs, H, W, acc ? 1 : 0);
+    if ((st = check_launch("tile gather", s)) != PTX_OK) return st;
+  }
+  return PTX_OK;
+}
+
+}  // namespace
+}  // namespace ptx
