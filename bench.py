@@ -42,12 +42,13 @@ def parse_args(argv=None):
     ap.add_argument("--plain-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the unfused torch path")
-    ap.add_argument("--deterministic", action="store_true", help="sorted-segment object gradient")
+    ap.add_argument("--atomic", action="store_true",
+                    help="object gradient by float atomics instead of the (default) sorted-segment gather")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args(argv)
 
 
-def kernel_bytes(label: str, B: int, N: int, H: int, W: int, nparts: int) -> float | None:
+def kernel_bytes(label: str, B: int, N: int, H: int, W: int, nparts: int, atomic: bool) -> float | None:
     """Algorithmic HBM bytes of one launch of each kernel of the fused step
     (N = probe pixels, B = positions in the launch)."""
     table = {
@@ -55,9 +56,12 @@ def kernel_bytes(label: str, B: int, N: int, H: int, W: int, nparts: int) -> flo
         "base gather rows": B * 16 * N + 8 * N,
         # read 8N + read measured 4N + write 8N per position
         "base step cols": B * 20 * N,
-        # read 8N + re-read sample window 8N + object read-modify-write 16N per position;
-        # probe read 8N and probe partials written nparts*8N once
-        "base scatter rows": B * 32 * N + 8 * N + nparts * 8 * N,
+        # read 8N + re-read sample window 8N per position, then either the object
+        # read-modify-write 16N (atomic) or the window contribution write 8N
+        # (sorted segments); probe read 8N and probe partials nparts*8N once
+        "base scatter rows": B * (32 if atomic else 24) * N + 8 * N + nparts * 8 * N,
+        # every window contribution read once (8N per position) + object written once
+        "tile gather": B * 8 * N + 8 * H * W,
         "probe partial sum": nparts * 8 * N + 8 * N,
         "zero fill": 8 * H * W,
     }
@@ -94,7 +98,7 @@ def main(argv=None) -> int:
     P = ds.positions.shape[0]
     nb = P // BATCH  # full mini-batches
     N = PROBE * PROBE
-    step = BaseStep(ds.sample.shape, ds.probe.shape, BATCH, dev, deterministic_mode=args.deterministic)
+    step = BaseStep(ds.sample.shape, ds.probe.shape, BATCH, dev, scatter="atomic" if args.atomic else "segments")
     sample = ds.sample.clone()
     probe = ds.probe.clone()
     gs = torch.empty_like(sample)
@@ -143,11 +147,11 @@ def main(argv=None) -> int:
     kernels = {}
     for label, v in per.items():
         avg = float(np.mean(v[1:] if len(v) > 2 else v))
-        nbytes = kernel_bytes(label, BATCH, N, OBJ, OBJ, nparts)
+        nbytes = kernel_bytes(label, BATCH, N, OBJ, OBJ, nparts, args.atomic)
         kernels[label] = {"avg_ms": round(avg, 5),
                           "gbs": round(nbytes / (avg * 1e-3) / 1e9, 1) if nbytes else None}
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
-    dom_bytes = kernel_bytes(dom, BATCH, N, OBJ, OBJ, nparts)
+    dom_bytes = kernel_bytes(dom, BATCH, N, OBJ, OBJ, nparts, args.atomic)
     dom_gbs = dom_bytes / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
     for label, d in kernels.items():
         print(f"[bench]   {label:22s} {d['avg_ms']:.4f} ms  {d['gbs']} GB/s", file=sys.stderr, flush=True)
@@ -229,7 +233,7 @@ def main(argv=None) -> int:
         "config": {"workload": "conventional ptychography fused fwd+loss+adjoint, object 2048x2048, probe 512x512, "
                                "900 positions, mini-batch 64",
                    "object": OBJ, "probe": PROBE, "positions": P, "mini_batch": BATCH,
-                   "object_gradient": "sorted-segment" if args.deterministic else "atomic"},
+                   "object_gradient": "atomic" if args.atomic else "sorted-segment"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": None},
         "path_roofline": {"bytes_per_pattern": b_alg, "achieved": round(path_gbs, 1), "unit": "GB/s",

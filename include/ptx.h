@@ -45,12 +45,19 @@ typedef enum ptx_status {
   PTX_ERR_HIP = 5
 } ptx_status;
 
-/* Object / spectrum gradient by a sorted-segment gather in ascending view
- * order instead of float atomics: bitwise reproducible, same summation order
- * as the reference torch loop (python/ptychoX/base/ops.py:100-105). */
+/* Object / spectrum gradient (conventional, near-field, FPM).  By default it is
+ * a sorted-segment gather: every view writes its window contribution once and
+ * each output tile then sums the windows that cover it in ascending view
+ * order -- bitwise reproducible, the summation order of the reference torch
+ * loop (python/ptychoX/base/ops.py:100-105), and faster than float atomics on
+ * MI355X.  PTX_FLAG_ATOMIC_SCATTER switches to float atomic adds (the
+ * reference CUDA kernels' method, base_kernel.cu:139-169); PTX_FLAG_DETERMINISTIC
+ * forces the sorted-segment gather even when PTX_FLAG_ATOMIC_SCATTER is set.
+ * The coded operator is deterministic in every mode. */
 #define PTX_FLAG_DETERMINISTIC 1u
 /* Add into the gradient outputs instead of overwriting them. */
 #define PTX_FLAG_ACCUMULATE 2u
+#define PTX_FLAG_ATOMIC_SCATTER 4u
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* ptx_last_error(void);
@@ -211,6 +218,47 @@ ptx_status ptx_nearfield_step(const void* sample, int64_t sample_h, int64_t samp
                               float* loss_per_position, float* amplitude,
                               void* grad_sample, void* grad_probe,
                               void* workspace, size_t workspace_bytes, unsigned flags, void* stream);
+
+/* ------------------------------------------------------------------ coded ptychography
+ * Workspace bytes of the coded entry points: step = 0 for ptx_coded_forward and
+ * ptx_coded_backward, step != 0 for ptx_coded_step. */
+size_t ptx_coded_workspace(int64_t height, int64_t width, int64_t batch, int64_t downsample, int step);
+
+/* Forward of ptychoX_coded on H x W frames:
+ *   object_spectrum = fft2(object_field);  R_b = exp(i 2 pi (dx_b fx + dy_b fy)) (signed integer f);
+ *   x_forward[b] = ifft2(object_spectrum transfer_1 R_b);
+ *   sensor_field[b] = ifft2(fft2(x_forward[b] coded_surface) transfer_2);
+ *   amplitude[b] = sqrt(max(avgpool_ds |sensor_field[b]|^2, 0))  -> [batch, H/ds, W/ds].
+ * shifts_xy: device float [batch][2] normalised shifts (dx, dy).
+ * Replaces ptychox_coded_forward_cuda (python/ptychoX/csrc/coded_kernel.cu:283-345, bound at
+ * coded_extension.cpp:26-56) and the torch path coded/ops.py:136-154.  x_forward and
+ * sensor_field are normalised like the torch path (the CUDA extension keeps them
+ * unnormalised, coded_kernel.cu:318,341). */
+ptx_status ptx_coded_forward(const void* object_field, int64_t height, int64_t width, const void* coded_surface,
+                             const void* transfer_1, const void* transfer_2, const float* shifts_xy, int64_t batch,
+                             int64_t downsample, void* object_spectrum, void* x_forward, void* sensor_field,
+                             float* amplitude, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Backward of ptychoX_coded w.r.t. object_field and coded_surface (transfer
+ * functions and shifts get no gradient, coded/ops.py:133):
+ *   gZ = Z G_up/(ds^2 max(A_up, 1e-7)); gY = ifft2(conj(H2) fft2(gZ));
+ *   grad_coded = sum_b gY conj(X_b); grad_object = N ifft2(sum_b conj(H1 R_b) fft2(gY conj(C))/N).
+ * Replaces ptychox_coded_backward_cuda (python/ptychoX/csrc/coded_kernel.cu:347-424, bound at
+ * coded_extension.cpp:58-88).  Always deterministic: both shared gradients are
+ * accumulated over fixed chunks of views and summed in a fixed order. */
+ptx_status ptx_coded_backward(const float* grad_output, const void* object_spectrum, const void* x_forward,
+                              const void* sensor_field, const void* coded_surface, const void* transfer_1,
+                              const void* transfer_2, const float* shifts_xy, int64_t batch, int64_t height,
+                              int64_t width, int64_t downsample, void* grad_object, void* grad_coded,
+                              void* workspace, size_t workspace_bytes, unsigned flags, void* stream);
+
+/* Fused coded training step with the amplitude MSE loss of python/train_coded.py:231-238
+ * (loss conventions of ptx_base_step; loss_per_position averages over the H/ds x W/ds pixels). */
+ptx_status ptx_coded_step(const void* object_field, int64_t height, int64_t width, const void* coded_surface,
+                          const void* transfer_1, const void* transfer_2, const float* shifts_xy, int64_t batch,
+                          int64_t downsample, const float* measured, float loss_scale, float* loss_per_position,
+                          float* amplitude, void* grad_object, void* grad_coded, void* workspace,
+                          size_t workspace_bytes, unsigned flags, void* stream);
 
 #ifdef __cplusplus
 }

@@ -6,9 +6,11 @@ near-field operator.  ``use_cuda_kernels=True`` on complex64 GPU tensors runs
 the hand-written HIP kernels of ``libptychox_hip.so``.
 """
 
-from ._utils import matlab_positions_to_torch, set_deterministic
+from ._utils import matlab_positions_to_torch, scatter_mode, set_scatter_mode
 from .base import clear_cuda_plan_cache as clear_base_cuda_plan_cache
 from .base import ptychoX_base, ptychography_base
+from .coded import clear_cuda_plan_cache as clear_coded_cuda_plan_cache
+from .coded import coded_ptychography, ptychoX_coded
 from .fpm import clear_cuda_plan_cache as clear_fpm_cuda_plan_cache
 from .fpm import fourier_ptychography, ptychoX_fourier
 from .nearfield import angular_spectrum_transfer, nearfield_ptychography, ptychoX_nearfield
@@ -19,6 +21,7 @@ def clear_cuda_plan_cache() -> None:
     """Kept for drop-in compatibility (python/ptychoX/__init__.py:10-13)."""
     clear_base_cuda_plan_cache()
     clear_fpm_cuda_plan_cache()
+    clear_coded_cuda_plan_cache()
     clear_nearfield_cuda_plan_cache()
 
 
@@ -26,6 +29,9 @@ __all__ = [
     "clear_cuda_plan_cache",
     "clear_base_cuda_plan_cache",
     "clear_fpm_cuda_plan_cache",
+    "clear_coded_cuda_plan_cache",
+    "coded_ptychography",
+    "ptychoX_coded",
     "clear_nearfield_cuda_plan_cache",
     "angular_spectrum_transfer",
     "nearfield_ptychography",
@@ -35,5 +41,6 @@ __all__ = [
     "matlab_positions_to_torch",
     "ptychoX_base",
     "ptychography_base",
-    "set_deterministic",
+    "scatter_mode",
+    "set_scatter_mode",
 ]

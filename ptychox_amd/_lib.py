@@ -17,6 +17,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libptychox_hip.so"
 
 PTX_FLAG_DETERMINISTIC = 1
 PTX_FLAG_ACCUMULATE = 2
+PTX_FLAG_ATOMIC_SCATTER = 4
 
 _STATUS = {
     0: "PTX_OK",
@@ -76,6 +77,16 @@ SIGNATURES: dict[str, tuple] = {
     "ptx_nearfield_step": (
         _st,
         [_p, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _p, _f, _p, _p, _p, _p, _p, _sz, _u, _p],
+    ),
+    "ptx_coded_workspace": (_sz, [_i64, _i64, _i64, _i64, _i]),
+    "ptx_coded_forward": (_st, [_p, _i64, _i64, _p, _p, _p, _p, _i64, _i64, _p, _p, _p, _p, _p, _sz, _p]),
+    "ptx_coded_backward": (
+        _st,
+        [_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _p, _p, _p, _sz, _u, _p],
+    ),
+    "ptx_coded_step": (
+        _st,
+        [_p, _i64, _i64, _p, _p, _p, _p, _i64, _i64, _p, _f, _p, _p, _p, _p, _p, _sz, _u, _p],
     ),
 }
 

@@ -13,7 +13,10 @@ import torch
 PTX_MIN_N = 8
 PTX_MAX_N = 2048
 
-_DETERMINISTIC = os.environ.get("PTYCHOX_DETERMINISTIC", "0") not in ("", "0", "false", "False")
+_SCATTER_MODES = ("segments", "atomic")
+_SCATTER = os.environ.get("PTYCHOX_SCATTER", "segments")
+if _SCATTER not in _SCATTER_MODES:
+    raise ValueError(f"PTYCHOX_SCATTER must be one of {_SCATTER_MODES}, got {_SCATTER!r}")
 
 
 def check_complex_scan_inputs(
@@ -77,16 +80,33 @@ def check_windows(positions_xy: torch.Tensor, h: int, w: int, H: int, W: int) ->
         )
 
 
-def deterministic() -> bool:
-    """Sorted-segment (atomic-free) object accumulation: on when
-    ``set_deterministic(True)``, ``PTYCHOX_DETERMINISTIC=1`` or
-    ``torch.use_deterministic_algorithms(True)``."""
-    return _DETERMINISTIC or torch.are_deterministic_algorithms_enabled()
+def set_scatter_mode(mode: str | None) -> None:
+    """How object / spectrum gradients are accumulated on the HIP path:
+    ``"segments"`` (default; sorted-segment gather in ascending view order,
+    bitwise reproducible) or ``"atomic"`` (float atomic adds).  ``None``
+    restores the default.  ``torch.use_deterministic_algorithms(True)``
+    always selects ``"segments"``."""
+    global _SCATTER
+    mode = "segments" if mode is None else mode
+    if mode not in _SCATTER_MODES:
+        raise ValueError(f"scatter mode must be one of {_SCATTER_MODES}, got {mode!r}")
+    _SCATTER = mode
 
 
-def set_deterministic(enabled: bool) -> None:
-    global _DETERMINISTIC
-    _DETERMINISTIC = bool(enabled)
+def scatter_mode() -> str:
+    return "segments" if torch.are_deterministic_algorithms_enabled() else _SCATTER
+
+
+def scatter_flags(mode: str | None = None) -> int:
+    """C-ABI flag bits for ``mode`` (default: the current :func:`scatter_mode`)."""
+    from ._lib import PTX_FLAG_ATOMIC_SCATTER, PTX_FLAG_DETERMINISTIC
+
+    if torch.are_deterministic_algorithms_enabled():
+        return PTX_FLAG_DETERMINISTIC
+    mode = scatter_mode() if mode is None else mode
+    if mode not in _SCATTER_MODES:
+        raise ValueError(f"scatter mode must be one of {_SCATTER_MODES}, got {mode!r}")
+    return PTX_FLAG_ATOMIC_SCATTER if mode == "atomic" else 0
 
 
 def matlab_positions_to_torch(positions_xy_1based: torch.Tensor) -> torch.Tensor:

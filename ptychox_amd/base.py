@@ -16,7 +16,7 @@ from ._utils import (
     can_use_native,
     check_complex_scan_inputs,
     check_windows,
-    deterministic,
+    scatter_flags,
     require_native_frame,
 )
 
@@ -100,7 +100,7 @@ def native_backward(grad_output, far_field, sample, probe, positions_xy, flags=N
     h, w = probe.shape
     B = positions_xy.shape[0]
     if flags is None:
-        flags = _lib.PTX_FLAG_DETERMINISTIC if deterministic() else 0
+        flags = scatter_flags()
     dev = sample.device
     nbytes = _lib.query("ptx_base_backward_workspace", H, W, h, w, B, flags)
     ws = _lib.workspace(nbytes, dev)
@@ -163,7 +163,7 @@ class BaseStep:
     the reference drivers (python/train_ptychography.py:555).
     """
 
-    def __init__(self, sample_shape, probe_shape, batch, device, deterministic_mode=None, keep_amplitude=False):
+    def __init__(self, sample_shape, probe_shape, batch, device, scatter=None, keep_amplitude=False):
         self.H, self.W = (int(s) for s in sample_shape)
         self.h, self.w = (int(s) for s in probe_shape)
         require_native_frame(self.h, self.w, "probe")
@@ -171,8 +171,7 @@ class BaseStep:
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
-        det = deterministic() if deterministic_mode is None else deterministic_mode
-        self.flags = _lib.PTX_FLAG_DETERMINISTIC if det else 0
+        self.flags = scatter_flags(scatter)
         nbytes = _lib.query("ptx_base_step_workspace", self.H, self.W, self.h, self.w, self.batch, self.flags)
         self.ws = _lib.workspace(nbytes, self.device)
         self.loss = torch.empty(self.batch, dtype=torch.float32, device=self.device)

@@ -134,17 +134,24 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
 #pragma unroll
   for (int j = 0; j < RPT; ++j) acc[j] = c_zero();
   const int n = counts[tile];
+  const int* list = lists + tile * batch;
   for (int k = 0; k < n; ++k) {
-    const int64_t b = lists[tile * batch + k];
+    const int64_t b = list[k];
+    // window origin in [0, H) x [0, W); offsets from it fold back by one
+    // conditional add (no integer division on the per-pixel path)
     const int64_t oy = pmod(pos[2 * b + 1] + off_y, H), ox = pmod(pos[2 * b] + off_x, W);
-    const int64_t cx = pmod(X - ox, W);
+    int64_t cx = X - ox;
+    if (cx < 0) cx += W;
     if (X >= W || cx >= w) continue;
     const float2* src = contrib + (b * h) * w + cx;
+    int64_t cy = Y0 - oy;
+    if (cy < 0) cy += H;
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
       const int64_t Y = Y0 + j * RSTEP;
-      const int64_t cy = pmod(Y - oy, H);
       if (Y < H && cy < h) acc[j] = c_add(acc[j], src[cy * w]);
+      cy += RSTEP;
+      if (cy >= H) cy -= H;
     }
   }
   if (X >= W) return;

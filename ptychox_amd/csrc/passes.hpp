@@ -92,6 +92,59 @@ inline void launch_cols(const Op& op, int64_t frames, int64_t width, hipStream_t
   }
 }
 
+// ---------------------------------------------------------------- transfer-function multiply
+// Applies F^-1_cols D F_cols to the columns of row-transformed frames: with the
+// rows already (inverse-)transformed, this completes  F^-1 (D . F x)  up to the
+// final row pass.  ``transfer`` is one [N, width] frame shared by every frame of
+// the batch (CONJ multiplies by its conjugate, for the adjoint).
+template <int N, int C, bool CONJ>
+__global__ void __launch_bounds__(ColCfg<N>::T * C)
+    k_transfer_cols(float2* __restrict__ tb, const float2* __restrict__ transfer, int64_t width,
+                    int64_t tiles_per_frame) {
+  using Cf = ColCfg<N>;
+  constexpr int E = Cf::E, T = Cf::T;
+  __shared__ float2 lds[C * lds_len<N>()];
+  const int c = threadIdx.x % C;
+  const int t = threadIdx.x / C;
+  const int64_t frame = blockIdx.x / tiles_per_frame;
+  const int64_t col = (blockIdx.x % tiles_per_frame) * C + c;
+  float2* p = tb + frame * N * width + col;
+  float2 v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = p[(int64_t)(t + q * T) * width];
+  fft_seq<N, E, -1>(v, t, &lds[c * lds_len<N>()]);
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const float2 tr = transfer[(int64_t)(t + q * T) * width + col];
+    v[q] = CONJ ? c_mulc(v[q], tr) : c_mul(v[q], tr);
+  }
+  __syncthreads();  // second transform reuses the LDS region
+  fft_seq<N, E, +1>(v, t, &lds[c * lds_len<N>()]);
+#pragma unroll
+  for (int q = 0; q < E; ++q) p[(int64_t)(t + q * T) * width] = v[q];
+}
+
+// host launcher (check_launch is the caller's)
+template <int N>
+inline void launch_transfer_cols(float2* tb, const float2* transfer, int64_t frames, int64_t width, bool conj,
+                                 hipStream_t s) {
+  using Cf = ColCfg<N>;
+  const int C = width >= Cf::CMAX ? Cf::CMAX : 8;
+  const int64_t tiles = width / C;
+  const dim3 grid((unsigned)(frames * tiles));
+  if (C == Cf::CMAX) {
+    if (conj)
+      k_transfer_cols<N, Cf::CMAX, true><<<grid, dim3(Cf::T * Cf::CMAX), 0, s>>>(tb, transfer, width, tiles);
+    else
+      k_transfer_cols<N, Cf::CMAX, false><<<grid, dim3(Cf::T * Cf::CMAX), 0, s>>>(tb, transfer, width, tiles);
+  } else {
+    if (conj)
+      k_transfer_cols<N, 8, true><<<grid, dim3(Cf::T * 8), 0, s>>>(tb, transfer, width, tiles);
+    else
+      k_transfer_cols<N, 8, false><<<grid, dim3(Cf::T * 8), 0, s>>>(tb, transfer, width, tiles);
+  }
+}
+
 // ---------------------------------------------------------------- plain ops
 
 // rows of a contiguous [nrows, N] complex array, in place, scale on store

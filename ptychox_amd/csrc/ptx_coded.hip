@@ -1,0 +1,750 @@
+// Coded ptychography (ptychoX_coded): forward, adjoint and the fused
+// amplitude-loss training step, full-frame H x W per view.
+//
+// Reference semantics: python/ptychoX/coded/ops.py:136-171 (forward under
+// autograd) and the CUDA extension python/ptychoX/csrc/coded_kernel.cu:283-424:
+//   O^ = F O;  R_b = exp(i 2 pi (dx_b fx + dy_b fy)), f = signed integer frequency
+//   X_b = F^-1(O^ H1 R_b);  Z_b = F^-1(F(X_b C) H2);  A_b = sqrt(max(pool_ds |Z_b|^2, 0))
+//   gZ = Z G_up/(ds^2 max(A_up, 1e-7));  gY = F^-1(conj(H2) F gZ)
+//   grad_C = sum_b gY conj(X_b);  gX = gY conj(C);  grad_O^ = sum_b conj(H1 R_b) F(gX_b)/N
+//   grad_O = N F^-1(grad_O^)
+//
+// The ramp is separable, R_b = ex_b[fx] ey_b[fy], with both factors computed in
+// double precision from the exact turn count (the reference evaluates the
+// argument 2 pi (dx fx + dy fy) in float32, coded_kernel.cu:117-122).
+//
+// Per view: P1 columns (O^ H1 R_b, column iFFT) | P2 rows (row iFFT -> X_b,
+// * C, row FFT) | P3 columns (FFT, * H2, iFFT) | P4 rows (row iFFT -> Z_b,
+// amplitude).  Backward: Q1 rows (gZ, row FFT) | Q2 columns (FFT, * conj H2,
+// iFFT) | Q3 rows over a chunk of views (row iFFT -> gY, mask partials, * conj C,
+// row FFT) | Q4 columns over a chunk of views (column FFT, spectrum partials),
+// then fixed-order partial sums and the object-size inverse FFT.  Both shared
+// gradients accumulate in registers over the chunk: no atomics anywhere, so
+// the operator is deterministic.
+#include "common_kernels.hpp"
+#include "passes.hpp"
+#include "ptx_internal.hpp"
+
+namespace ptx {
+namespace {
+
+constexpr float kCodedAmpEps = 1.0e-7f;  // python/ptychoX/csrc/coded_kernel.cu:227
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+// ex[b, kx] = exp(i 2 pi dx_b fx),  ey[b, ky] = exp(i 2 pi dy_b fy)   (tables [B, W] and [B, H])
+static __global__ void __launch_bounds__(256) k_ramp_tables(const float* __restrict__ shifts, int64_t B, int64_t H,
+                                                            int64_t W, float2* __restrict__ ex,
+                                                            float2* __restrict__ ey) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t per = H + W;
+  if (i >= B * per) return;
+  const int64_t b = i / per, k = i % per;
+  const bool is_x = k < W;
+  const int64_t n = is_x ? W : H;
+  const int64_t j = is_x ? k : k - W;
+  const int64_t f = 2 * j >= n ? j - n : j;
+  const double turns = (double)shifts[2 * b + (is_x ? 0 : 1)] * (double)f;
+  const double frac = turns - rint(turns);
+  double sn, cs;
+  sincos(kTwoPi * frac, &sn, &cs);
+  const float2 v = make_float2((float)cs, (float)sn);
+  if (is_x)
+    ex[b * W + j] = v;
+  else
+    ey[b * H + j] = v;
+}
+
+// P1: O^ H1 R_b -> column iFFT
+struct CodedSpecCols {
+  const float2* spec;
+  const float2* h1;
+  const float2* ex;
+  const float2* ey;
+  float2* out;
+  int64_t H, width;
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    const float2 rx = ex[frame * width + col];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int64_t r = t + q * T;
+      const int64_t k = r * width + col;
+      v[q] = c_mul(c_mul(spec[k], h1[k]), c_mul(ey[frame * H + r], rx));
+    }
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    float2* p = out + frame * N * width + col;
+#pragma unroll
+    for (int q = 0; q < E; ++q) p[(int64_t)(t + q * T) * width] = v[q];
+  }
+};
+
+// P2: row iFFT -> X = v/N (stored), * C, row FFT (in place)
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_coded_x_rows(float2* __restrict__ tb, float2* __restrict__ xfwd, const float2* __restrict__ mask, int64_t H,
+                   int64_t nrows, float inv_n) {
+  using Cf = RowCfg<N>;
+  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  __shared__ float2 lds[S * lds_len<N>()];
+  const int seq = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int64_t row = (int64_t)blockIdx.x * S + seq;
+  const bool valid = row < nrows;
+  const int64_t y = valid ? row % H : 0;
+  float2 v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = valid ? tb[row * N + t + q * T] : c_zero();
+  fft_seq<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int i = t + q * T;
+    const float2 x = c_scale(v[q], inv_n);
+    if (valid) xfwd[row * N + i] = x;
+    v[q] = c_mul(x, mask[y * N + i]);
+  }
+  __syncthreads();
+  fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) tb[row * N + t + q * T] = v[q];
+  }
+}
+
+// P4 (forward): row iFFT -> Z = v/N stored; ds == 1: amplitude |Z|
+struct CodedZRows {
+  const float2* in;
+  float2* z;
+  float* amp;  // null when ds > 1 (pooled separately)
+  float inv_n;
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = valid ? in[row * N + t + q * T] : c_zero();
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    if (!valid) return;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int64_t k = row * N + t + q * T;
+      const float2 e = c_scale(v[q], inv_n);
+      z[k] = e;
+      if (amp) amp[k] = c_abs(e);
+    }
+  }
+};
+
+// RMS pooling of |Z|^2 over ds x ds cells.  mode 0: amplitude only;
+// mode 1 (backward): sc = G/(ds^2 max(A, eps)); mode 2 (step): amplitude, loss
+// partial per pooled row and sc = gscale (A - Y)/(ds^2 max(A, eps)).
+static __global__ void __launch_bounds__(256)
+    k_coded_pool(const float2* __restrict__ z, int64_t B, int64_t H, int64_t W, int ds, const float* __restrict__ g,
+                 float* __restrict__ amp, float* __restrict__ sc, float* __restrict__ loss_part, float gscale,
+                 int mode) {
+  const int64_t lh = H / ds, lw = W / ds;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= B * lh * lw) return;
+  const int64_t b = i / (lh * lw), rem = i % (lh * lw);
+  const int64_t ly = rem / lw, lx = rem % lw;
+  float s = 0.0f;
+  for (int yy = 0; yy < ds; ++yy)
+    for (int xx = 0; xx < ds; ++xx) {
+      const float2 v = z[(b * H + ly * ds + yy) * W + lx * ds + xx];
+      s += v.x * v.x + v.y * v.y;
+    }
+  const float inv_cell = 1.0f / (float)(ds * ds);
+  const float a = sqrtf(fmaxf(s * inv_cell, 0.0f));
+  if (amp) amp[i] = a;
+  if (mode == 1) {
+    sc[i] = g[i] * inv_cell / fmaxf(a, kCodedAmpEps);
+  } else if (mode == 2) {
+    const float d = a - g[i];
+    sc[i] = gscale * d * inv_cell / fmaxf(a, kCodedAmpEps);
+    loss_part[i] = d * d;
+  }
+}
+
+// Q1: gZ = Z * sc_up (ds > 1, scale map from k_coded_pool) or
+// gZ = Z G/max(|Z|, eps) (ds == 1, straight from the upstream gradient); row FFT -> out
+struct CodedGradZRows {
+  const float2* z;
+  const float* sc;  // ds > 1: [B, H/ds, W/ds] scale map;  ds == 1: G [B, H, W]
+  float2* out;
+  int64_t H;
+  int ds;
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    const int64_t b = valid ? row / H : 0, y = valid ? row % H : 0;
+    const int64_t lw = N / ds;
+    const float* srow = sc + (b * (H / ds) + y / ds) * lw;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int i = t + q * T;
+      if (!valid) {
+        v[q] = c_zero();
+      } else if (ds == 1) {
+        const float2 e = z[row * N + i];
+        v[q] = c_scale(e, srow[i] / fmaxf(c_abs(e), kCodedAmpEps));
+      } else {
+        v[q] = c_scale(z[row * N + i], srow[i / ds]);
+      }
+    }
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t row, bool valid, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+    if (!valid) return;
+#pragma unroll
+    for (int q = 0; q < E; ++q) out[row * N + t + q * T] = v[q];
+  }
+};
+
+// P4 + Q1 fused for the ds == 1 step: row iFFT -> Z, A = |Z|, loss partial per
+// row, gZ = Z gscale (A - Y)/max(A, eps), row FFT (in place)
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_coded_step_rows(float2* __restrict__ tb, const float* __restrict__ measured, float* __restrict__ amp_out,
+                      float* __restrict__ loss_part, int64_t nrows, float gscale, float inv_n) {
+  using Cf = RowCfg<N>;
+  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  __shared__ float2 lds[S * lds_len<N>()];
+  __shared__ float red[256];
+  const int seq = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int64_t row = (int64_t)blockIdx.x * S + seq;
+  const bool valid = row < nrows;
+  float2 v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = valid ? tb[row * N + t + q * T] : c_zero();
+  fft_seq<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
+  float lsum = 0.0f;
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int64_t k = row * N + t + q * T;
+    const float2 e = c_scale(v[q], inv_n);
+    const float a = c_abs(e);
+    const float d = valid ? a - measured[k] : 0.0f;
+    lsum += d * d;
+    if (valid && amp_out) amp_out[k] = a;
+    v[q] = c_scale(e, gscale * d / fmaxf(a, kCodedAmpEps));
+  }
+  red[threadIdx.x] = lsum;
+  __syncthreads();
+  if (valid && t == 0) {
+    float sum = 0.0f;
+    for (int k = 0; k < T; ++k) sum += red[seq * T + k];
+    loss_part[row] = sum;
+  }
+  fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) tb[row * N + t + q * T] = v[q];
+  }
+}
+
+// Q3: per (row group, chunk of views): row iFFT -> gY = v/N; mask partial += gY conj(X_b);
+// gX = gY conj(C); row FFT (in place)
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_coded_y_rows(float2* __restrict__ tb, const float2* __restrict__ xfwd, const float2* __restrict__ mask,
+                   int64_t H, int64_t B, int chunk, int ngroups, float inv_n, float2* __restrict__ gc_part) {
+  using Cf = RowCfg<N>;
+  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  __shared__ float2 lds[S * lds_len<N>()];
+  const int seq = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int group = blockIdx.x % ngroups;
+  const int ch = blockIdx.x / ngroups;
+  const int64_t y = (int64_t)group * S + seq;
+  const bool yv = y < H;
+  const int64_t b0 = (int64_t)ch * chunk;
+  const int64_t b1 = b0 + chunk < B ? b0 + chunk : B;
+  float2 cm[E], acc[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    cm[q] = yv ? mask[y * N + t + q * T] : c_zero();
+    acc[q] = c_zero();
+  }
+  for (int64_t b = b0; b < b1; ++b) {
+    const int64_t row = b * H + y;
+    float2 v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = yv ? tb[row * N + t + q * T] : c_zero();
+    fft_seq<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const float2 gy = c_scale(v[q], inv_n);
+      if (yv) acc[q] = c_add(acc[q], c_mulc(gy, xfwd[row * N + t + q * T]));
+      v[q] = c_mulc(gy, cm[q]);
+    }
+    __syncthreads();
+    fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
+    if (yv) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) tb[row * N + t + q * T] = v[q];
+    }
+    __syncthreads();
+  }
+  if (yv) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) gc_part[((int64_t)ch * H + y) * N + t + q * T] = acc[q];
+  }
+}
+
+// Q4: per (column tile, chunk of views): column FFT; spectrum partial += conj(H1 R_b) v / N
+template <int N, int C>
+__global__ void __launch_bounds__(ColCfg<N>::T * C)
+    k_coded_spec_cols(const float2* __restrict__ tb, const float2* __restrict__ h1, const float2* __restrict__ ex,
+                      const float2* __restrict__ ey, int64_t B, int64_t width, int64_t tiles, int chunk, float inv_n,
+                      float2* __restrict__ gs_part) {
+  using Cf = ColCfg<N>;
+  constexpr int E = Cf::E, T = Cf::T;
+  __shared__ float2 lds[C * lds_len<N>()];
+  const int c = threadIdx.x % C;
+  const int t = threadIdx.x / C;
+  const int64_t tile = blockIdx.x % tiles;
+  const int ch = blockIdx.x / tiles;
+  const int64_t col = tile * C + c;
+  const int64_t b0 = (int64_t)ch * chunk;
+  const int64_t b1 = b0 + chunk < B ? b0 + chunk : B;
+  float2 hc[E], acc[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    hc[q] = h1[(int64_t)(t + q * T) * width + col];
+    acc[q] = c_zero();
+  }
+  for (int64_t b = b0; b < b1; ++b) {
+    const float2* p = tb + b * N * width + col;
+    float2 v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = p[(int64_t)(t + q * T) * width];
+    fft_seq<N, E, -1>(v, t, &lds[c * lds_len<N>()]);
+    const float2 rx = ex[b * width + col];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const float2 r = c_mul(ey[b * N + t + q * T], rx);
+      // conj(H1 R) v / N
+      acc[q] = c_add(acc[q], c_scale(c_mulc(c_mulc(v[q], hc[q]), r), inv_n));
+    }
+    __syncthreads();
+  }
+  float2* d = gs_part + (int64_t)ch * N * width + col;
+#pragma unroll
+  for (int q = 0; q < E; ++q) d[(int64_t)(t + q * T) * width] = acc[q];
+}
+
+// ---------------------------------------------------------------- host launchers
+
+template <int N>
+struct SpecColsRun {
+  static ptx_status run(const CodedSpecCols& op, int64_t frames, hipStream_t s) {
+    launch_cols<N, +1>(op, frames, op.width, s);
+    return check_launch("coded spectrum cols", s);
+  }
+};
+template <int N>
+struct XRowsRun {
+  static ptx_status run(float2* tb, float2* xfwd, const float2* mask, int64_t H, int64_t nrows, float inv_n,
+                        hipStream_t s) {
+    const int64_t blocks = (nrows + RowCfg<N>::S - 1) / RowCfg<N>::S;
+    k_coded_x_rows<N><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(tb, xfwd, mask, H, nrows, inv_n);
+    return check_launch("coded x rows", s);
+  }
+};
+template <int N>
+struct TransferRun {
+  static ptx_status run(float2* tb, const float2* tr, int64_t frames, int64_t width, bool conj, hipStream_t s) {
+    launch_transfer_cols<N>(tb, tr, frames, width, conj, s);
+    return check_launch(conj ? "coded transfer* cols" : "coded transfer cols", s);
+  }
+};
+template <int N>
+struct ZRowsRun {
+  static ptx_status run(const CodedZRows& op, int64_t nrows, hipStream_t s) {
+    launch_rows<N, +1>(op, nrows, s);
+    return check_launch("coded z rows", s);
+  }
+};
+template <int N>
+struct GradZRowsRun {
+  static ptx_status run(const CodedGradZRows& op, int64_t nrows, hipStream_t s) {
+    launch_rows<N, -1>(op, nrows, s);
+    return check_launch("coded grad-z rows", s);
+  }
+};
+template <int N>
+struct StepRowsRun {
+  static ptx_status run(float2* tb, const float* measured, float* amp, float* loss_part, int64_t nrows, float gscale,
+                        float inv_n, hipStream_t s) {
+    const int64_t blocks = (nrows + RowCfg<N>::S - 1) / RowCfg<N>::S;
+    k_coded_step_rows<N><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(tb, measured, amp, loss_part, nrows, gscale,
+                                                                      inv_n);
+    return check_launch("coded step rows", s);
+  }
+};
+
+struct Chunks {
+  int row_groups, row_chunk, row_nchunks;  // Q3
+  int col_tiles, col_c, col_chunk, col_nchunks;  // Q4
+};
+
+template <int N>
+struct RowChunkQuery {
+  static ptx_status run(int64_t H, int64_t B, Chunks* c) {
+    c->row_groups = (int)((H + RowCfg<N>::S - 1) / RowCfg<N>::S);
+    int64_t want = (2048 + c->row_groups - 1) / c->row_groups;
+    want = want < 1 ? 1 : (want > B ? B : want);
+    c->row_chunk = (int)((B + want - 1) / want);
+    c->row_nchunks = (int)((B + c->row_chunk - 1) / c->row_chunk);
+    return PTX_OK;
+  }
+};
+template <int N>
+struct ColChunkQuery {
+  static ptx_status run(int64_t W, int64_t B, Chunks* c) {
+    c->col_c = W >= ColCfg<N>::CMAX ? ColCfg<N>::CMAX : 8;
+    c->col_tiles = (int)(W / c->col_c);
+    int64_t want = (2048 + c->col_tiles - 1) / c->col_tiles;
+    want = want < 1 ? 1 : (want > B ? B : want);
+    c->col_chunk = (int)((B + want - 1) / want);
+    c->col_nchunks = (int)((B + c->col_chunk - 1) / c->col_chunk);
+    return PTX_OK;
+  }
+};
+template <int N>
+struct YRowsRun {
+  static ptx_status run(float2* tb, const float2* xfwd, const float2* mask, int64_t H, int64_t B, const Chunks& c,
+                        float inv_n, float2* gc_part, hipStream_t s) {
+    k_coded_y_rows<N><<<dim3((unsigned)(c.row_groups * c.row_nchunks)), dim3(256), 0, s>>>(
+        tb, xfwd, mask, H, B, c.row_chunk, c.row_groups, inv_n, gc_part);
+    return check_launch("coded y rows", s);
+  }
+};
+template <int N>
+struct SpecGradColsRun {
+  static ptx_status run(const float2* tb, const float2* h1, const float2* ex, const float2* ey, int64_t B, int64_t W,
+                        const Chunks& c, float inv_n, float2* gs_part, hipStream_t s) {
+    const dim3 grid((unsigned)(c.col_tiles * c.col_nchunks));
+    if (c.col_c == ColCfg<N>::CMAX) {
+      constexpr int C = ColCfg<N>::CMAX;
+      k_coded_spec_cols<N, C><<<grid, dim3(ColCfg<N>::T * C), 0, s>>>(tb, h1, ex, ey, B, W, c.col_tiles, c.col_chunk,
+                                                                       inv_n, gs_part);
+    } else {
+      k_coded_spec_cols<N, 8><<<grid, dim3(ColCfg<N>::T * 8), 0, s>>>(tb, h1, ex, ey, B, W, c.col_tiles, c.col_chunk,
+                                                                       inv_n, gs_part);
+    }
+    return check_launch("coded spectrum-grad cols", s);
+  }
+};
+template <int N>
+struct RowsOOPRun {
+  static ptx_status run(const float2* in, float2* out, int64_t nrows, int dir, hipStream_t s) {
+    RowsOutOfPlace op{in, out, 1.0f};
+    if (dir < 0)
+      launch_rows<N, -1>(op, nrows, s);
+    else
+      launch_rows<N, +1>(op, nrows, s);
+    return check_launch(dir < 0 ? "fft rows" : "ifft rows", s);
+  }
+};
+// columns of one [N, width] frame from `in` into `out` (= or +=)
+struct ColsInto {
+  const float2* in;
+  float2* out;
+  int64_t width;
+  int accumulate;
+  template <int N, int E>
+  __device__ __forceinline__ void load(int64_t, int64_t col, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = in[(int64_t)(t + q * T) * width + col];
+  }
+  template <int N, int E>
+  __device__ __forceinline__ void store(int64_t, int64_t col, int t, float2 (&v)[E]) const {
+    constexpr int T = N / E;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      float2* d = out + (int64_t)(t + q * T) * width + col;
+      *d = accumulate ? c_add(*d, v[q]) : v[q];
+    }
+  }
+};
+template <int N>
+struct ColsIntoRun {
+  static ptx_status run(const ColsInto& op, int dir, hipStream_t s) {
+    if (dir < 0)
+      launch_cols<N, -1>(op, 1, op.width, s);
+    else
+      launch_cols<N, +1>(op, 1, op.width, s);
+    return check_launch(dir < 0 ? "fft cols" : "ifft cols", s);
+  }
+};
+
+ptx_status fft2_frame(const float2* in, float2* out, int64_t H, int64_t W, int dir, int accumulate, float2* scratch,
+                      hipStream_t s) {
+  // rows out of place into scratch, columns from scratch into out
+  ptx_status st = dispatch_n<RowsOOPRun>(W, in, scratch, H, dir, s);
+  if (st != PTX_OK) return st;
+  ColsInto c{scratch, out, W, accumulate};
+  return dispatch_n<ColsIntoRun>(H, c, dir, s);
+}
+
+struct CodedWorkspace {
+  float2* spec;     // [H, W] (step)
+  float2* ex;       // [B, W]
+  float2* ey;       // [B, H]
+  float2* xfwd;     // [B, H, W] (step)
+  float2* tb;       // [B, H, W]
+  float2* gc_part;  // [row chunks, H, W]
+  float2* gs_part;  // [col chunks, H, W]
+  float2* gspec;    // [H, W]
+  float* sc;        // [B, H/ds, W/ds] (ds > 1)
+  float* loss_part; // step: [B * H] (ds == 1) or [B * H/ds * W/ds]
+  size_t bytes;
+};
+
+CodedWorkspace carve(void* base, int64_t H, int64_t W, int64_t B, int64_t ds, bool step, const Chunks& c) {
+  Carver cv{(char*)base, 0};
+  CodedWorkspace ws{};
+  ws.spec = step ? cv.take<float2>((size_t)(H * W)) : nullptr;
+  ws.ex = cv.take<float2>((size_t)(B * W));
+  ws.ey = cv.take<float2>((size_t)(B * H));
+  ws.xfwd = step ? cv.take<float2>((size_t)(B * H * W)) : nullptr;
+  ws.tb = cv.take<float2>((size_t)(B * H * W));
+  ws.gc_part = cv.take<float2>((size_t)(c.row_nchunks * H * W));
+  ws.gs_part = cv.take<float2>((size_t)(c.col_nchunks * H * W));
+  ws.gspec = cv.take<float2>((size_t)(H * W));
+  const int64_t low = B * (H / ds) * (W / ds);
+  ws.sc = ds > 1 ? cv.take<float>((size_t)low) : nullptr;
+  ws.loss_part = step ? cv.take<float>((size_t)(ds > 1 ? low : B * H)) : nullptr;
+  ws.bytes = cv.used;
+  return ws;
+}
+
+ptx_status check(const char* fn, int64_t H, int64_t W, int64_t B, int64_t ds) {
+  if (B < 0) {
+    set_error("%s: negative batch", fn);
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  if (!size_ok(H) || !size_ok(W)) {
+    set_error("%s: frame %lld x %lld unsupported (sides must be powers of two in [%d, %d])", fn, (long long)H,
+              (long long)W, PTX_MIN_N, PTX_MAX_N);
+    return PTX_ERR_UNSUPPORTED;
+  }
+  if (ds < 1 || H % ds != 0 || W % ds != 0) {
+    set_error("%s: object_field shape must be divisible by downsample (%lld)", fn, (long long)ds);
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  return PTX_OK;
+}
+
+Chunks plan(int64_t H, int64_t W, int64_t B) {
+  Chunks c{};
+  dispatch_n<RowChunkQuery>(W, H, B, &c);
+  dispatch_n<ColChunkQuery>(H, W, B, &c);
+  return c;
+}
+
+ptx_status ramps(const float* shifts, int64_t B, int64_t H, int64_t W, const CodedWorkspace& ws, hipStream_t s) {
+  const int64_t n = B * (H + W);
+  k_ramp_tables<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(shifts, B, H, W, ws.ex, ws.ey);
+  return check_launch("coded ramp tables", s);
+}
+
+// P1..P3 of the forward: spectrum -> X (stored) -> T holds F_cols^-1 (F(X C) H2) row-transformed
+ptx_status forward_to_t(const float2* spec, const float2* mask, const float2* t1, const float2* t2, int64_t H,
+                        int64_t W, int64_t B, const CodedWorkspace& ws, float2* xfwd, hipStream_t s) {
+  const float inv_n = 1.0f / (float)(H * W);
+  CodedSpecCols p1{spec, t1, ws.ex, ws.ey, ws.tb, H, W};
+  ptx_status st = dispatch_n<SpecColsRun>(H, p1, B, s);
+  if (st != PTX_OK) return st;
+  if ((st = dispatch_n<XRowsRun>(W, ws.tb, xfwd, mask, H, B * H, inv_n, s)) != PTX_OK) return st;
+  return dispatch_n<TransferRun>(H, ws.tb, t2, B, W, false, s);
+}
+
+// Q2..end: T holds row-FFT of gZ; produces grad_object and grad_coded
+ptx_status backward_from_t(const float2* xfwd, const float2* mask, const float2* t1, const float2* t2, int64_t H,
+                           int64_t W, int64_t B, const Chunks& c, const CodedWorkspace& ws, float2* gobj,
+                           float2* gcoded, int accumulate, hipStream_t s) {
+  const float inv_n = 1.0f / (float)(H * W);
+  ptx_status st = dispatch_n<TransferRun>(H, ws.tb, t2, B, W, true, s);
+  if (st != PTX_OK) return st;
+  if ((st = dispatch_n<YRowsRun>(W, ws.tb, xfwd, mask, H, B, c, inv_n, ws.gc_part, s)) != PTX_OK) return st;
+  if ((st = dispatch_n<SpecGradColsRun>(H, ws.tb, t1, ws.ex, ws.ey, B, W, c, inv_n, ws.gs_part, s)) != PTX_OK)
+    return st;
+  sum_parts2(ws.gc_part, c.row_nchunks, H * W, gcoded, accumulate, s);
+  if ((st = check_launch("mask partial sum", s)) != PTX_OK) return st;
+  sum_parts2(ws.gs_part, c.col_nchunks, H * W, ws.gspec, 0, s);
+  if ((st = check_launch("spectrum partial sum", s)) != PTX_OK) return st;
+  // grad_O = N ifft2(grad_O^) = unnormalised inverse DFT; rows in place, columns into the output
+  return fft2_frame(ws.gspec, gobj, H, W, +1, accumulate, ws.gspec, s);
+}
+
+}  // namespace
+}  // namespace ptx
+
+using namespace ptx;
+
+extern "C" {
+
+ptx_status ptx_coded_forward(const void* object_field, int64_t H, int64_t W, const void* coded_surface,
+                             const void* transfer_1, const void* transfer_2, const float* shifts_xy, int64_t B,
+                             int64_t downsample, void* object_spectrum, void* x_forward, void* sensor_field,
+                             float* amplitude, void* workspace, size_t ws_bytes, void* stream) {
+  ptx_status st = check("ptx_coded_forward", H, W, B, downsample);
+  if (st != PTX_OK) return st;
+  if (!object_field || !object_spectrum || (B > 0 && (!coded_surface || !transfer_1 || !transfer_2 || !shifts_xy ||
+                                                      !x_forward || !sensor_field || !amplitude))) {
+    set_error("ptx_coded_forward: null pointer argument");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  const Chunks c = plan(H, W, B);
+  const CodedWorkspace ws = carve(workspace, H, W, B, downsample, false, c);
+  if (B > 0 && (!workspace || ws_bytes < ws.bytes)) {
+    set_error("ptx_coded_forward: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
+    return PTX_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  timing_begin(s);
+  float2* spec = (float2*)object_spectrum;
+  if ((st = fft2_frame((const float2*)object_field, spec, H, W, -1, 0, spec, s)) != PTX_OK) return st;
+  if (B == 0) return PTX_OK;
+  if ((st = ramps(shifts_xy, B, H, W, ws, s)) != PTX_OK) return st;
+  if ((st = forward_to_t(spec, (const float2*)coded_surface, (const float2*)transfer_1, (const float2*)transfer_2, H,
+                         W, B, ws, (float2*)x_forward, s)) != PTX_OK)
+    return st;
+  CodedZRows p4{ws.tb, (float2*)sensor_field, downsample == 1 ? amplitude : nullptr, 1.0f / (float)(H * W)};
+  if ((st = dispatch_n<ZRowsRun>(W, p4, B * H, s)) != PTX_OK) return st;
+  if (downsample > 1) {
+    const int64_t n = B * (H / downsample) * (W / downsample);
+    k_coded_pool<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(
+        (const float2*)sensor_field, B, H, W, (int)downsample, nullptr, amplitude, nullptr, nullptr, 0.0f, 0);
+    if ((st = check_launch("coded pool", s)) != PTX_OK) return st;
+  }
+  return PTX_OK;
+}
+
+size_t ptx_coded_workspace(int64_t H, int64_t W, int64_t B, int64_t downsample, int step) {
+  if (check("ptx_coded_workspace", H, W, B, downsample) != PTX_OK) return 0;
+  return carve(nullptr, H, W, B, downsample, step != 0, plan(H, W, B)).bytes;
+}
+
+ptx_status ptx_coded_backward(const float* grad_output, const void* object_spectrum, const void* x_forward,
+                              const void* sensor_field, const void* coded_surface, const void* transfer_1,
+                              const void* transfer_2, const float* shifts_xy, int64_t B, int64_t H, int64_t W,
+                              int64_t downsample, void* grad_object, void* grad_coded, void* workspace,
+                              size_t ws_bytes, unsigned flags, void* stream) {
+  ptx_status st = check("ptx_coded_backward", H, W, B, downsample);
+  if (st != PTX_OK) return st;
+  if (!grad_object || !grad_coded) {
+    set_error("ptx_coded_backward: null gradient output");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  timing_begin(s);
+  const int acc = (flags & PTX_FLAG_ACCUMULATE) ? 1 : 0;
+  if (B == 0) {
+    if (!acc) {
+      fill_zero2((float2*)grad_object, H * W, s);
+      fill_zero2((float2*)grad_coded, H * W, s);
+    }
+    return check_launch("zero fill", s);
+  }
+  if (!grad_output || !object_spectrum || !x_forward || !sensor_field || !coded_surface || !transfer_1 ||
+      !transfer_2 || !shifts_xy) {
+    set_error("ptx_coded_backward: null pointer argument");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  const Chunks c = plan(H, W, B);
+  const CodedWorkspace ws = carve(workspace, H, W, B, downsample, false, c);
+  if (!workspace || ws_bytes < ws.bytes) {
+    set_error("ptx_coded_backward: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
+    return PTX_ERR_WORKSPACE;
+  }
+  if ((st = ramps(shifts_xy, B, H, W, ws, s)) != PTX_OK) return st;
+  const float* scmap = grad_output;
+  if (downsample > 1) {
+    // scale map G/(ds^2 max(A, eps)) from the recomputed amplitude (coded_kernel.cu:370-377)
+    const int64_t n = B * (H / downsample) * (W / downsample);
+    k_coded_pool<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(
+        (const float2*)sensor_field, B, H, W, (int)downsample, grad_output, nullptr, ws.sc, nullptr, 0.0f, 1);
+    if ((st = check_launch("coded pool", s)) != PTX_OK) return st;
+    scmap = ws.sc;
+  }
+  CodedGradZRows q1{(const float2*)sensor_field, scmap, ws.tb, H, (int)downsample};
+  if ((st = dispatch_n<GradZRowsRun>(W, q1, B * H, s)) != PTX_OK) return st;
+  return backward_from_t((const float2*)x_forward, (const float2*)coded_surface, (const float2*)transfer_1,
+                         (const float2*)transfer_2, H, W, B, c, ws, (float2*)grad_object, (float2*)grad_coded, acc, s);
+}
+
+ptx_status ptx_coded_step(const void* object_field, int64_t H, int64_t W, const void* coded_surface,
+                          const void* transfer_1, const void* transfer_2, const float* shifts_xy, int64_t B,
+                          int64_t downsample, const float* measured, float loss_scale, float* loss_per_position,
+                          float* amplitude, void* grad_object, void* grad_coded, void* workspace, size_t ws_bytes,
+                          unsigned flags, void* stream) {
+  ptx_status st = check("ptx_coded_step", H, W, B, downsample);
+  if (st != PTX_OK) return st;
+  if (!grad_object || !grad_coded) {
+    set_error("ptx_coded_step: null gradient output");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  timing_begin(s);
+  const int acc = (flags & PTX_FLAG_ACCUMULATE) ? 1 : 0;
+  if (B == 0) {
+    if (!acc) {
+      fill_zero2((float2*)grad_object, H * W, s);
+      fill_zero2((float2*)grad_coded, H * W, s);
+    }
+    return check_launch("zero fill", s);
+  }
+  if (!object_field || !coded_surface || !transfer_1 || !transfer_2 || !shifts_xy || !measured ||
+      !loss_per_position) {
+    set_error("ptx_coded_step: null pointer argument");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  const Chunks c = plan(H, W, B);
+  const CodedWorkspace ws = carve(workspace, H, W, B, downsample, true, c);
+  if (!workspace || ws_bytes < ws.bytes) {
+    set_error("ptx_coded_step: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
+    return PTX_ERR_WORKSPACE;
+  }
+  const int64_t ds = downsample;
+  const int64_t lh = H / ds, lw = W / ds;
+  if ((st = fft2_frame((const float2*)object_field, ws.spec, H, W, -1, 0, ws.spec, s)) != PTX_OK) return st;
+  if ((st = ramps(shifts_xy, B, H, W, ws, s)) != PTX_OK) return st;
+  if ((st = forward_to_t(ws.spec, (const float2*)coded_surface, (const float2*)transfer_1, (const float2*)transfer_2,
+                         H, W, B, ws, ws.xfwd, s)) != PTX_OK)
+    return st;
+  const float inv_n = 1.0f / (float)(H * W);
+  if (ds == 1) {
+    if ((st = dispatch_n<StepRowsRun>(W, ws.tb, measured, amplitude, ws.loss_part, B * H, 2.0f * loss_scale, inv_n,
+                                      s)) != PTX_OK)
+      return st;
+    loss_rows(ws.loss_part, H, B, inv_n, loss_per_position, s);
+    if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
+  } else {
+    // Z in place in T, then pooled amplitude / loss / scale map, then gZ row FFT
+    CodedZRows p4{ws.tb, ws.tb, nullptr, inv_n};
+    if ((st = dispatch_n<ZRowsRun>(W, p4, B * H, s)) != PTX_OK) return st;
+    const int64_t n = B * lh * lw;
+    k_coded_pool<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(ws.tb, B, H, W, (int)ds, measured, amplitude,
+                                                                        ws.sc, ws.loss_part, 2.0f * loss_scale, 2);
+    if ((st = check_launch("coded pool", s)) != PTX_OK) return st;
+    loss_rows(ws.loss_part, lh * lw, B, 1.0f / (float)(lh * lw), loss_per_position, s);
+    if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
+    CodedGradZRows q1{ws.tb, ws.sc, ws.tb, H, (int)ds};
+    if ((st = dispatch_n<GradZRowsRun>(W, q1, B * H, s)) != PTX_OK) return st;
+  }
+  return backward_from_t(ws.xfwd, (const float2*)coded_surface, (const float2*)transfer_1, (const float2*)transfer_2,
+                         H, W, B, c, ws, (float2*)grad_object, (float2*)grad_coded, acc, s);
+}
+
+}  // extern "C"

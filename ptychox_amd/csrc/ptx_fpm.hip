@@ -360,7 +360,7 @@ FpmWorkspace carve(void* base, int64_t H, int64_t W, int64_t h, int64_t w, int64
   ws.tb = cv.take<float2>((size_t)(B * h * w));
   ws.gq_part = cv.take<float2>((size_t)(p.nchunks * h * w));
   ws.loss_part = step ? cv.take<float>((size_t)(B * tiles)) : nullptr;
-  if (flags & PTX_FLAG_DETERMINISTIC) {
+  if (use_segments(flags)) {
     const TileGrid tg = tile_grid(H, W);
     ws.lists = cv.take<int>((size_t)(tg.count() * B));
     ws.counts = cv.take<int>((size_t)tg.count());
@@ -399,7 +399,7 @@ ptx_status spectrum_of(const float2* obj, float2* spec, int64_t H, int64_t W, hi
 ptx_status fpm_tail(const FpmWorkspace& ws, const Plan& p, const float2* spec, int64_t H, int64_t W,
                     const float2* pupil, int64_t h, int64_t w, const int64_t* led, int64_t B, float2* gobj,
                     float2* gpupil, unsigned flags, hipStream_t s) {
-  const bool det = flags & PTX_FLAG_DETERMINISTIC;
+  const bool det = use_segments(flags);
   const bool acc = flags & PTX_FLAG_ACCUMULATE;
   ptx_status st = PTX_OK;
   if (!det) {

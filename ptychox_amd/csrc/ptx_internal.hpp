@@ -16,6 +16,11 @@ namespace ptx {
 
 void set_error(const char* fmt, ...);
 
+// sorted-segment gather unless atomics are requested (and not overridden)
+inline bool use_segments(unsigned flags) {
+  return (flags & PTX_FLAG_DETERMINISTIC) || !(flags & PTX_FLAG_ATOMIC_SCATTER);
+}
+
 inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 inline bool size_ok(int64_t n) { return is_pow2(n) && n >= PTX_MIN_N && n <= PTX_MAX_N; }
 

@@ -19,34 +19,6 @@
 namespace ptx {
 namespace {
 
-// column FFT, * T (or conj T), column iFFT -- in place
-template <int N, int C, bool CONJ>
-__global__ void __launch_bounds__(ColCfg<N>::T * C)
-    k_transfer_cols(float2* __restrict__ tb, const float2* __restrict__ transfer, int64_t width,
-                    int64_t tiles_per_frame) {
-  using Cf = ColCfg<N>;
-  constexpr int E = Cf::E, T = Cf::T;
-  __shared__ float2 lds[C * lds_len<N>()];
-  const int c = threadIdx.x % C;
-  const int t = threadIdx.x / C;
-  const int64_t frame = blockIdx.x / tiles_per_frame;
-  const int64_t col = (blockIdx.x % tiles_per_frame) * C + c;
-  float2* p = tb + frame * N * width + col;
-  float2 v[E];
-#pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = p[(int64_t)(t + q * T) * width];
-  fft_seq<N, E, -1>(v, t, &lds[c * lds_len<N>()]);
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const float2 tr = transfer[(int64_t)(t + q * T) * width + col];
-    v[q] = CONJ ? c_mulc(v[q], tr) : c_mul(v[q], tr);
-  }
-  __syncthreads();  // second transform reuses the LDS region
-  fft_seq<N, E, +1>(v, t, &lds[c * lds_len<N>()]);
-#pragma unroll
-  for (int q = 0; q < E; ++q) p[(int64_t)(t + q * T) * width] = v[q];
-}
-
 // row iFFT / N -> field and amplitude
 struct NfFieldRows {
   const float2* in;
@@ -149,21 +121,7 @@ __global__ void __launch_bounds__(256)
 template <int N>
 struct TransferColsRun {
   static ptx_status run(float2* tb, const float2* transfer, int64_t frames, int64_t width, bool conj, hipStream_t s) {
-    using Cf = ColCfg<N>;
-    const int C = width >= Cf::CMAX ? Cf::CMAX : 8;
-    const int64_t tiles = width / C;
-    const dim3 grid((unsigned)(frames * tiles));
-    if (C == Cf::CMAX) {
-      if (conj)
-        k_transfer_cols<N, Cf::CMAX, true><<<grid, dim3(Cf::T * Cf::CMAX), 0, s>>>(tb, transfer, width, tiles);
-      else
-        k_transfer_cols<N, Cf::CMAX, false><<<grid, dim3(Cf::T * Cf::CMAX), 0, s>>>(tb, transfer, width, tiles);
-    } else {
-      if (conj)
-        k_transfer_cols<N, 8, true><<<grid, dim3(Cf::T * 8), 0, s>>>(tb, transfer, width, tiles);
-      else
-        k_transfer_cols<N, 8, false><<<grid, dim3(Cf::T * 8), 0, s>>>(tb, transfer, width, tiles);
-    }
+    launch_transfer_cols<N>(tb, transfer, frames, width, conj, s);
     return check_launch(conj ? "nf transfer* cols" : "nf transfer cols", s);
   }
 };

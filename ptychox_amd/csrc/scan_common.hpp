@@ -175,7 +175,7 @@ ScanWorkspace carve_scan(void* base, int64_t H, int64_t W, int64_t h, int64_t w,
   ws.tb = cv.take<float2>((size_t)(B * h * w));
   ws.gp_part = cv.take<float2>((size_t)(sp.nchunks * h * w));
   ws.loss_part = step ? cv.take<float>((size_t)(B * col_tiles)) : nullptr;
-  if (flags & PTX_FLAG_DETERMINISTIC) {
+  if (use_segments(flags)) {
     const TileGrid tg = tile_grid(H, W);
     ws.lists = cv.take<int>((size_t)(tg.count() * B));
     ws.counts = cv.take<int>((size_t)tg.count());
@@ -207,7 +207,7 @@ template <bool SIGN>
 ptx_status scan_scatter(const ScanWorkspace& ws, const ScatterPlan& sp, const float2* sample, int64_t H, int64_t W,
                         const float2* probe, int64_t h, int64_t w, const int64_t* pos, int64_t B, float2* gs,
                         float2* gp, unsigned flags, float scale, hipStream_t s) {
-  const bool det = flags & PTX_FLAG_DETERMINISTIC;
+  const bool det = use_segments(flags);
   const bool acc = flags & PTX_FLAG_ACCUMULATE;
   ptx_status st = PTX_OK;
   if (!det && !acc) {

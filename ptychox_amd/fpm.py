@@ -11,7 +11,7 @@ from __future__ import annotations
 import torch
 
 from . import _lib
-from ._utils import can_use_native, check_complex_scan_inputs, deterministic, require_native_frame
+from ._utils import can_use_native, check_complex_scan_inputs, require_native_frame, scatter_flags
 
 
 def ptychoX_fourier(
@@ -95,7 +95,7 @@ def native_backward(grad_output, image_field, object_fft, pupil, led, flags=None
     B = led.shape[0]
     dev = pupil.device
     if flags is None:
-        flags = _lib.PTX_FLAG_DETERMINISTIC if deterministic() else 0
+        flags = scatter_flags()
     ws = _lib.workspace(_lib.query("ptx_fpm_backward_workspace", H, W, h, w, B, flags), dev)
     grad_object = torch.empty((H, W), dtype=torch.complex64, device=dev)
     grad_pupil = torch.empty((h, w), dtype=torch.complex64, device=dev)
@@ -147,7 +147,7 @@ class FpmStep:
     """Fused FPM training step on the HIP path (``ptx_fpm_step``): object FFT,
     forward, amplitude MSE loss and both gradients, reusable workspace."""
 
-    def __init__(self, object_shape, pupil_shape, batch, device, deterministic_mode=None, keep_amplitude=False):
+    def __init__(self, object_shape, pupil_shape, batch, device, scatter=None, keep_amplitude=False):
         self.H, self.W = (int(s) for s in object_shape)
         self.h, self.w = (int(s) for s in pupil_shape)
         require_native_frame(self.H, self.W, "object_field")
@@ -156,8 +156,7 @@ class FpmStep:
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
-        det = deterministic() if deterministic_mode is None else deterministic_mode
-        self.flags = _lib.PTX_FLAG_DETERMINISTIC if det else 0
+        self.flags = scatter_flags(scatter)
         nbytes = _lib.query("ptx_fpm_step_workspace", self.H, self.W, self.h, self.w, self.batch, self.flags)
         self.ws = _lib.workspace(nbytes, self.device)
         self.loss = torch.empty(self.batch, dtype=torch.float32, device=self.device)

@@ -19,7 +19,7 @@ from ._utils import (
     can_use_native,
     check_complex_scan_inputs,
     check_windows,
-    deterministic,
+    scatter_flags,
     require_native_frame,
 )
 from .base import crop_indices
@@ -113,7 +113,7 @@ def native_backward(grad_output, field, sample, probe, transfer, positions_xy, f
     B = positions_xy.shape[0]
     dev = sample.device
     if flags is None:
-        flags = _lib.PTX_FLAG_DETERMINISTIC if deterministic() else 0
+        flags = scatter_flags()
     ws = _lib.workspace(_lib.query("ptx_nearfield_backward_workspace", H, W, h, w, B, flags), dev)
     gs = torch.empty((H, W), dtype=torch.complex64, device=dev)
     gp = torch.empty((h, w), dtype=torch.complex64, device=dev)
@@ -162,7 +162,7 @@ class PtychoXNearFieldFunction(torch.autograd.Function):
 class NearFieldStep:
     """Fused near-field training step on the HIP path (``ptx_nearfield_step``)."""
 
-    def __init__(self, sample_shape, probe_shape, batch, device, deterministic_mode=None, keep_amplitude=False):
+    def __init__(self, sample_shape, probe_shape, batch, device, scatter=None, keep_amplitude=False):
         self.H, self.W = (int(s) for s in sample_shape)
         self.h, self.w = (int(s) for s in probe_shape)
         require_native_frame(self.h, self.w, "probe")
@@ -170,8 +170,7 @@ class NearFieldStep:
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
-        det = deterministic() if deterministic_mode is None else deterministic_mode
-        self.flags = _lib.PTX_FLAG_DETERMINISTIC if det else 0
+        self.flags = scatter_flags(scatter)
         nbytes = _lib.query("ptx_nearfield_step_workspace", self.H, self.W, self.h, self.w, self.batch, self.flags)
         self.ws = _lib.workspace(nbytes, self.device)
         self.loss = torch.empty(self.batch, dtype=torch.float32, device=self.device)

@@ -69,9 +69,13 @@ def test_unsupported_sizes_are_rejected_without_gpu(lib):
 def test_workspace_queries(lib):
     n = lib.query("ptx_base_backward_workspace", 2048, 2048, 512, 512, 64, 0)
     assert n >= 64 * 512 * 512 * 8
-    n_det = lib.query("ptx_base_backward_workspace", 2048, 2048, 512, 512, 64, 1)
-    assert n_det > n
+    # the default (sorted segments) needs the tile lists on top of the atomic mode's buffers
+    n_atomic = lib.query("ptx_base_backward_workspace", 2048, 2048, 512, 512, 64, lib.PTX_FLAG_ATOMIC_SCATTER)
+    assert n > n_atomic
+    assert lib.query("ptx_base_backward_workspace", 2048, 2048, 512, 512, 64, lib.PTX_FLAG_DETERMINISTIC) == n
     assert lib.query("ptx_base_step_workspace", 2048, 2048, 512, 512, 64, 0) > n
+    assert lib.query("ptx_coded_workspace", 1024, 1024, 49, 1, 1) > lib.query("ptx_coded_workspace", 1024, 1024, 49, 1, 0)
+    assert lib.query("ptx_coded_workspace", 1024, 1024, 49, 3, 0) == 0  # 1024 not divisible by 3
     assert lib.query("ptx_base_backward_workspace", 2048, 2048, 500, 512, 64, 0) == 0
 
 

@@ -68,14 +68,14 @@ def test_base_forward_backward_vs_oracle(gpu, H, h, B):
 @pytest.mark.parametrize("det", [False, True])
 def test_base_native_vs_torch_path_full_size(gpu, det):
     """BASELINE configuration 2 shapes: object 2048^2, probe 512^2, B=64."""
-    from ptychox_amd import ptychoX_base, set_deterministic
+    from ptychox_amd import ptychoX_base, set_scatter_mode
 
     rng = np.random.default_rng(2)
     s = smooth_object(rng, 2048, 4.0)
     p = disc_probe(512, 128)
     pos = raster_positions(rng, 30, 52, 14, 3)[:64]
     g = rng.standard_normal((64, 512, 512)).astype(np.float32)
-    set_deterministic(det)
+    set_scatter_mode("segments" if det else "atomic")
     try:
         outs = []
         for native in (True, False):
@@ -85,19 +85,19 @@ def test_base_native_vs_torch_path_full_size(gpu, det):
             amp.backward(to_dev(g, gpu))
             outs.append((amp.detach(), st.grad, pt.grad))
     finally:
-        set_deterministic(False)
+        set_scatter_mode(None)
     e = [nrel(a, b) for a, b in zip(outs[0], outs[1])]
     print(f"base 2048/512/B64 det={det}: amp {e[0]:.2e} grad_sample {e[1]:.2e} grad_probe {e[2]:.2e} (vs torch path)", flush=True)
     assert max(e) < TOL
 
 
 def test_base_deterministic_bitwise(gpu):
-    from ptychox_amd import ptychoX_base, set_deterministic
+    from ptychox_amd import ptychoX_base, set_scatter_mode
 
     rng = np.random.default_rng(3)
     s, p, pos = _case(rng, 400, 128, 24, gpu)
     g = to_dev(rng.standard_normal((24, 128, 128)).astype(np.float32), gpu)
-    set_deterministic(True)
+    set_scatter_mode("segments")
     try:
         grads = []
         for _ in range(2):
@@ -106,7 +106,7 @@ def test_base_deterministic_bitwise(gpu):
             ptychoX_base(st, pt, to_dev(pos, gpu), use_cuda_kernels=True).backward(g)
             grads.append((st.grad.clone(), pt.grad.clone()))
     finally:
-        set_deterministic(False)
+        set_scatter_mode(None)
     assert torch.equal(grads[0][0], grads[1][0])
     assert torch.equal(grads[0][1], grads[1][1])
 
@@ -119,7 +119,7 @@ def test_base_step_vs_oracle(gpu, det):
     s, p, pos = _case(rng, 320, 128, 10, gpu)
     y = np.abs(po.base_forward(s * 1.05, p, pos)["amplitude"]).astype(np.float32)
     loss, per, gs, gp = po.base_train_step(s, p, pos, y)
-    step = BaseStep(s.shape, p.shape, 10, gpu, deterministic_mode=det, keep_amplitude=True)
+    step = BaseStep(s.shape, p.shape, 10, gpu, scatter="segments" if det else "atomic", keep_amplitude=True)
     gst = torch.empty(s.shape, dtype=torch.complex64, device=gpu)
     gpt = torch.empty(p.shape, dtype=torch.complex64, device=gpu)
     lb = step(to_dev(s, gpu), to_dev(p, gpu), to_dev(pos, gpu), to_dev(y, gpu), gst, gpt)

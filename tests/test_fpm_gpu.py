@@ -41,7 +41,7 @@ def _leds(n_side, pitch, center, H):
 @pytest.mark.parametrize("H,h,B,det", [(16, 8, 3, False), (64, 32, 9, False), (256, 64, 25, False),
                                        (256, 64, 25, True), (128, 128, 4, False)])
 def test_fpm_vs_oracle(gpu, H, h, B, det):
-    from ptychox_amd import ptychoX_fourier, set_deterministic
+    from ptychox_amd import ptychoX_fourier, set_scatter_mode
 
     rng = np.random.default_rng(H + h + B)
     o = _rough_object(rng, H)
@@ -52,12 +52,12 @@ def test_fpm_vs_oracle(gpu, H, h, B, det):
     g = rng.standard_normal((B, h, h)).astype(np.float32)
     ot = to_dev(o, gpu).requires_grad_(True)
     qt = to_dev(q, gpu).requires_grad_(True)
-    set_deterministic(det)
+    set_scatter_mode("segments" if det else "atomic")
     try:
         amp = ptychoX_fourier(ot, qt, to_dev(led, gpu), use_cuda_kernels=True)
         amp.backward(to_dev(g, gpu))
     finally:
-        set_deterministic(False)
+        set_scatter_mode(None)
     fwd = po.fpm_forward(o, q, led)
     go, gq = po.fpm_backward(g, o.shape, q, fwd["sub_spectrum"], fwd["image_field"], led)
     e = (nrel(amp, fwd["amplitude"]), nrel(ot.grad, go), nrel(qt.grad, gq))
@@ -117,7 +117,7 @@ def test_fpm_step_vs_oracle(gpu, det):
     led = _leds(5, 6, 48, 128)
     y = po.fpm_forward(o * 1.1, q, led)["amplitude"].astype(np.float32)
     loss, per, go, gq = po.fpm_train_step(o, q, led, y)
-    step = FpmStep(o.shape, q.shape, 25, gpu, deterministic_mode=det)
+    step = FpmStep(o.shape, q.shape, 25, gpu, scatter="segments" if det else "atomic")
     got = torch.empty(o.shape, dtype=torch.complex64, device=gpu)
     gqt = torch.empty(q.shape, dtype=torch.complex64, device=gpu)
     lb = step(to_dev(o, gpu), to_dev(q, gpu), to_dev(led, gpu), to_dev(y, gpu), got, gqt)

@@ -33,7 +33,7 @@ def _transfer(n, fresnel=20.0):
 @pytest.mark.parametrize("H,h,B,det", [(64, 32, 4, False), (256, 128, 8, False), (256, 128, 8, True),
                                        (400, 256, 5, False)])
 def test_nearfield_vs_oracle(gpu, H, h, B, det):
-    from ptychox_amd import ptychoX_nearfield, set_deterministic
+    from ptychox_amd import ptychoX_nearfield, set_scatter_mode
 
     rng = np.random.default_rng(H + h + B)
     s = smooth_object(rng, H)
@@ -43,12 +43,12 @@ def test_nearfield_vs_oracle(gpu, H, h, B, det):
     g = rng.standard_normal((B, h, h)).astype(np.float32)
     st = to_dev(s, gpu).requires_grad_(True)
     pt = to_dev(p, gpu).requires_grad_(True)
-    set_deterministic(det)
+    set_scatter_mode("segments" if det else "atomic")
     try:
         amp = ptychoX_nearfield(st, pt, to_dev(t, gpu), to_dev(pos, gpu), use_cuda_kernels=True)
         amp.backward(to_dev(g, gpu))
     finally:
-        set_deterministic(False)
+        set_scatter_mode(None)
     fwd = po.nearfield_forward(s, p, t, pos)
     gs, gp = po.nearfield_backward(g, s.shape, p, t, fwd["patches"], fwd["field"], pos)
     e = (nrel(amp, fwd["amplitude"]), nrel(st.grad, gs), nrel(pt.grad, gp))
@@ -91,7 +91,7 @@ def test_nearfield_step_vs_oracle(gpu, det):
     pos = rng.integers(0, 200 - 64 + 1, (12, 2)).astype(np.int64)
     y = po.nearfield_forward(s * 1.1, p, t, pos)["amplitude"].astype(np.float32)
     loss, per, gs, gp = po.nearfield_train_step(s, p, t, pos, y)
-    step = NearFieldStep(s.shape, p.shape, 12, gpu, deterministic_mode=det)
+    step = NearFieldStep(s.shape, p.shape, 12, gpu, scatter="segments" if det else "atomic")
     gst = torch.empty(s.shape, dtype=torch.complex64, device=gpu)
     gpt = torch.empty(p.shape, dtype=torch.complex64, device=gpu)
     lb = step(to_dev(s, gpu), to_dev(p, gpu), to_dev(t, gpu), to_dev(pos, gpu), to_dev(y, gpu), gst, gpt)
