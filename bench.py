@@ -45,6 +45,7 @@ def parse_args(argv=None):
     ap.add_argument("--atomic", action="store_true",
                     help="object gradient by float atomics instead of the (default) sorted-segment gather")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in FFT schedule (skip the autotuner)")
     return ap.parse_args(argv)
 
 
@@ -110,7 +111,14 @@ def main(argv=None) -> int:
         pos, meas = batches[k % nb]
         step(sample, probe, pos, meas, gs, gp)
 
-    # ---- warm-up, then the timed region
+    # ---- warm-up (including the FFT-schedule autotuner), then the timed region
+    tuning = None
+    if not args.no_tune:
+        from ptychox_amd import autotune
+
+        res = autotune.autotune(lambda: run(0), autotune.keys_for_scan((PROBE, PROBE)), reps=8)
+        tuning = {"table": res["table"], "before_ms": res["before_ms"], "after_ms": res["after_ms"]}
+        print(f"[bench] autotune: {tuning}", file=sys.stderr, flush=True)
     for k in range(args.warmup):
         run(k)
     torch.cuda.synchronize()
@@ -239,6 +247,7 @@ def main(argv=None) -> int:
         "path_roofline": {"bytes_per_pattern": b_alg, "achieved": round(path_gbs, 1), "unit": "GB/s",
                           "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
+        "tuning": tuning,
         "plain_path": plain,
         "cpu_baseline": cpu,
     }

@@ -78,6 +78,18 @@ ptx_status ptx_timing_events(void* const* events, int n);
 int ptx_timing_count(void);
 const char* ptx_timing_label(int k);
 
+/* ------------------------------------------------------------------ tuning table
+ * No counterpart in the reference (cuFFT chose its own kernels).  Every FFT pass
+ * is compiled for 8 and for 16 complex elements per thread (the radix of its
+ * in-register butterflies); this table picks one per pass kind and transform
+ * length.  kind: 0 row passes, 1 row passes that loop over a chunk of views
+ * (object / probe / mask accumulation), 2 column passes, 3 column passes that
+ * loop over views.  elems: 8 or 16, 0 restores the built-in default.
+ * Process-wide; set it before launching work (the autotuner in
+ * ptychox_amd/autotune.py measures and fills it). */
+ptx_status ptx_tune_set(int kind, int64_t n, int elems);
+int ptx_tune_get(int kind, int64_t n);
+
 /* ------------------------------------------------------------------ FFT
  * Batched in-place 2-D C2C transform of ``batch`` frames of h x w,
  * unnormalised in both directions (inverse != 0 uses exp(+i)).

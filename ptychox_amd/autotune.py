@@ -1,0 +1,120 @@
+"""Autotuner for the FFT pass schedule.
+
+Every FFT pass of libptychox_hip.so is compiled for 8 and for 16 complex
+elements per thread (radix-8 or radix-16 butterflies in registers, 64 or 32
+threads per 512-point sequence).  Which one is faster depends on the pass kind
+(plain row / column pass, or a pass that loops over views to accumulate a
+shared gradient) and the transform length.  :func:`autotune` times a real
+workload step for each candidate, one table entry at a time (coordinate
+descent, median of repeated hipEvent timings), and leaves the fastest setting
+in the library's process-wide table (``ptx_tune_set``).  Results can be saved
+to JSON and re-applied (``PTYCHOX_TUNING=<file>`` applies a file at import).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+from typing import Callable, Iterable
+
+import torch
+
+from . import _lib
+
+PASS_ROW, PASS_ROW_LOOP, PASS_COL, PASS_COL_LOOP = 0, 1, 2, 3
+KIND_NAMES = {PASS_ROW: "row", PASS_ROW_LOOP: "row_loop", PASS_COL: "col", PASS_COL_LOOP: "col_loop"}
+CANDIDATES = (8, 16)
+
+
+def get(kind: int, n: int) -> int:
+    return int(_lib.lib().ptx_tune_get(kind, n))
+
+
+def set_entry(kind: int, n: int, elems: int) -> None:
+    _lib.call("ptx_tune_set", kind, n, elems)
+
+
+def table(keys: Iterable[tuple[int, int]]) -> dict[str, int]:
+    return {f"{KIND_NAMES[k]}:{n}": get(k, n) for k, n in keys}
+
+
+def apply(entries: dict[str, int]) -> None:
+    """Apply ``{"row:512": 8, ...}`` (the format :func:`table` returns)."""
+    inv = {v: k for k, v in KIND_NAMES.items()}
+    for key, elems in entries.items():
+        kind, n = key.split(":")
+        set_entry(inv[kind], int(n), int(elems))
+
+
+def save(path, entries: dict[str, int]) -> None:
+    Path(path).write_text(json.dumps(entries, indent=1, sort_keys=True))
+
+
+def load(path) -> dict[str, int]:
+    entries = json.loads(Path(path).read_text())
+    apply(entries)
+    return entries
+
+
+def time_ms(run: Callable[[], None], reps: int = 10, warmup: int = 2) -> float:
+    """Median GPU time of ``run()`` over ``reps`` repetitions (hipEvents)."""
+    for _ in range(warmup):
+        run()
+    times = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        run()
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    times.sort()
+    return times[len(times) // 2]
+
+
+def autotune(run: Callable[[], None], keys: Iterable[tuple[int, int]], reps: int = 10,
+             rounds: int = 1) -> dict:
+    """Pick, for each ``(kind, n)`` in ``keys``, the elements-per-thread setting
+    that minimises the time of ``run()``; the table keeps the winners.
+
+    Returns ``{"table": {...}, "before_ms": t0, "after_ms": t1, "trials": [...]}``.
+    """
+    keys = [(int(k), int(n)) for k, n in keys if int(n) > 8]
+    before = time_ms(run, reps)
+    trials = []
+    for _ in range(rounds):
+        for kind, n in keys:
+            best, best_t = get(kind, n), None
+            for e in CANDIDATES:
+                set_entry(kind, n, e)
+                t = time_ms(run, reps)
+                trials.append({"key": f"{KIND_NAMES[kind]}:{n}", "elems": e, "ms": round(t, 5)})
+                if best_t is None or t < best_t:
+                    best, best_t = e, t
+            set_entry(kind, n, best)
+    after = time_ms(run, reps)
+    return {"table": table(keys), "before_ms": round(before, 5), "after_ms": round(after, 5), "trials": trials}
+
+
+def keys_for_scan(probe_shape) -> list[tuple[int, int]]:
+    """Table entries a conventional / near-field step uses."""
+    h, w = probe_shape
+    return [(PASS_ROW, w), (PASS_ROW_LOOP, w), (PASS_COL, h)]
+
+
+def keys_for_fpm(object_shape, pupil_shape) -> list[tuple[int, int]]:
+    H, W = object_shape
+    h, w = pupil_shape
+    return sorted({(PASS_ROW, w), (PASS_ROW, W), (PASS_ROW_LOOP, w), (PASS_COL, h), (PASS_COL, H)})
+
+
+def keys_for_coded(shape) -> list[tuple[int, int]]:
+    H, W = shape
+    return [(PASS_ROW, W), (PASS_ROW_LOOP, W), (PASS_COL, H), (PASS_COL_LOOP, H)]
+
+
+_env = os.environ.get("PTYCHOX_TUNING")
+if _env and Path(_env).exists():
+    load(_env)

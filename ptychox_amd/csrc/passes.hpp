@@ -20,17 +20,19 @@
 
 namespace ptx {
 
-template <int N>
+// E = complex elements per thread (the FFT radix); the runtime tuning table
+// (ptx_internal.hpp) picks it per pass kind and length among compiled variants.
+template <int N, int E = elems_for<N>()>
 struct RowCfg {
-  static constexpr int E = elems_for<N>();
+  static_assert(E <= N && N % E == 0, "E must divide N");
   static constexpr int T = N / E;
   static constexpr int THREADS = 256;
   static constexpr int S = THREADS / T;  // rows per work-group
 };
 
-template <int N>
+template <int N, int E = elems_for<N>()>
 struct ColCfg {
-  static constexpr int E = elems_for<N>();
+  static_assert(E <= N && N % E == 0, "E must divide N");
   static constexpr int T = N / E;
   // LDS budget per work-group ~72 KB (two work-groups per CU), <= 1024 threads
   static constexpr int BY_LDS = 73728 / (8 * lds_len<N>());
@@ -39,10 +41,10 @@ struct ColCfg {
   static constexpr int CMAX = CM0 >= 16 ? 16 : CM0 >= 8 ? 8 : CM0 >= 4 ? 4 : 2;
 };
 
-template <int N, int DIR, class Op>
+template <int N, int E, int DIR, class Op>
 __global__ void __launch_bounds__(256) k_rows(Op op, int64_t nrows) {
-  using Cf = RowCfg<N>;
-  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  using Cf = RowCfg<N, E>;
+  constexpr int T = Cf::T, S = Cf::S;
   __shared__ float2 lds[S * lds_len<N>()];
   const int seq = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -54,10 +56,8 @@ __global__ void __launch_bounds__(256) k_rows(Op op, int64_t nrows) {
   op.template store<N, E>(row, valid, t, v);
 }
 
-template <int N, int C, int DIR, class Op>
-__global__ void __launch_bounds__(ColCfg<N>::T * C) k_cols(Op op, int64_t tiles_per_frame) {
-  using Cf = ColCfg<N>;
-  constexpr int E = Cf::E;
+template <int N, int E, int C, int DIR, class Op>
+__global__ void __launch_bounds__((ColCfg<N, E>::T * C)) k_cols(Op op, int64_t tiles_per_frame) {
   __shared__ float2 lds[C * lds_len<N>()];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
@@ -70,25 +70,25 @@ __global__ void __launch_bounds__(ColCfg<N>::T * C) k_cols(Op op, int64_t tiles_
 }
 
 // Launch helpers (grid sizes); return false if the launch configuration is impossible.
-template <int N, int DIR, class Op>
+template <int N, int E, int DIR, class Op>
 inline void launch_rows(const Op& op, int64_t nrows, hipStream_t stream) {
-  using Cf = RowCfg<N>;
+  using Cf = RowCfg<N, E>;
   const int64_t blocks = (nrows + Cf::S - 1) / Cf::S;
-  k_rows<N, DIR, Op><<<dim3((unsigned)blocks), dim3(Cf::THREADS), 0, stream>>>(op, nrows);
+  k_rows<N, E, DIR, Op><<<dim3((unsigned)blocks), dim3(Cf::THREADS), 0, stream>>>(op, nrows);
 }
 
-template <int N, int DIR, class Op>
+template <int N, int E, int DIR, class Op>
 inline void launch_cols(const Op& op, int64_t frames, int64_t width, hipStream_t stream) {
-  using Cf = ColCfg<N>;
+  using Cf = ColCfg<N, E>;
   if (width >= Cf::CMAX) {
     constexpr int C = Cf::CMAX;
     const int64_t tiles = width / C;
-    k_cols<N, C, DIR, Op><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, stream>>>(op, tiles);
+    k_cols<N, E, C, DIR, Op><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, stream>>>(op, tiles);
   } else {
     // narrow frames (width 8 with CMAX 16): one tile of 8 columns per frame
     constexpr int C = 8;
     const int64_t tiles = width / C;
-    k_cols<N, C, DIR, Op><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, stream>>>(op, tiles);
+    k_cols<N, E, C, DIR, Op><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, stream>>>(op, tiles);
   }
 }
 
@@ -97,12 +97,12 @@ inline void launch_cols(const Op& op, int64_t frames, int64_t width, hipStream_t
 // rows already (inverse-)transformed, this completes  F^-1 (D . F x)  up to the
 // final row pass.  ``transfer`` is one [N, width] frame shared by every frame of
 // the batch (CONJ multiplies by its conjugate, for the adjoint).
-template <int N, int C, bool CONJ>
-__global__ void __launch_bounds__(ColCfg<N>::T * C)
+template <int N, int E, int C, bool CONJ>
+__global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     k_transfer_cols(float2* __restrict__ tb, const float2* __restrict__ transfer, int64_t width,
                     int64_t tiles_per_frame) {
-  using Cf = ColCfg<N>;
-  constexpr int E = Cf::E, T = Cf::T;
+  using Cf = ColCfg<N, E>;
+  constexpr int T = Cf::T;
   __shared__ float2 lds[C * lds_len<N>()];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
@@ -125,23 +125,23 @@ __global__ void __launch_bounds__(ColCfg<N>::T * C)
 }
 
 // host launcher (check_launch is the caller's)
-template <int N>
+template <int N, int E>
 inline void launch_transfer_cols(float2* tb, const float2* transfer, int64_t frames, int64_t width, bool conj,
                                  hipStream_t s) {
-  using Cf = ColCfg<N>;
+  using Cf = ColCfg<N, E>;
   const int C = width >= Cf::CMAX ? Cf::CMAX : 8;
   const int64_t tiles = width / C;
   const dim3 grid((unsigned)(frames * tiles));
   if (C == Cf::CMAX) {
     if (conj)
-      k_transfer_cols<N, Cf::CMAX, true><<<grid, dim3(Cf::T * Cf::CMAX), 0, s>>>(tb, transfer, width, tiles);
+      k_transfer_cols<N, E, Cf::CMAX, true><<<grid, dim3(Cf::T * Cf::CMAX), 0, s>>>(tb, transfer, width, tiles);
     else
-      k_transfer_cols<N, Cf::CMAX, false><<<grid, dim3(Cf::T * Cf::CMAX), 0, s>>>(tb, transfer, width, tiles);
+      k_transfer_cols<N, E, Cf::CMAX, false><<<grid, dim3(Cf::T * Cf::CMAX), 0, s>>>(tb, transfer, width, tiles);
   } else {
     if (conj)
-      k_transfer_cols<N, 8, true><<<grid, dim3(Cf::T * 8), 0, s>>>(tb, transfer, width, tiles);
+      k_transfer_cols<N, E, 8, true><<<grid, dim3(Cf::T * 8), 0, s>>>(tb, transfer, width, tiles);
     else
-      k_transfer_cols<N, 8, false><<<grid, dim3(Cf::T * 8), 0, s>>>(tb, transfer, width, tiles);
+      k_transfer_cols<N, E, 8, false><<<grid, dim3(Cf::T * 8), 0, s>>>(tb, transfer, width, tiles);
   }
 }
 

@@ -73,12 +73,12 @@ struct BaseGammaCols {
 };
 
 // ---------------------------------------------------------------- C12: fused forward/loss/adjoint column pass
-template <int N, int C>
-__global__ void __launch_bounds__(ColCfg<N>::T * C)
+template <int N, int E, int C>
+__global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     k_base_step_cols(float2* __restrict__ tb, const float* __restrict__ measured, float* __restrict__ amp_out,
                      float* __restrict__ loss_part, int64_t width, int64_t tiles_per_frame, float gscale) {
-  using Cf = ColCfg<N>;
-  constexpr int E = Cf::E, T = Cf::T;
+  using Cf = ColCfg<N, E>;
+  constexpr int T = Cf::T;
   __shared__ float2 lds[C * lds_len<N>()];
   __shared__ float red[16];
   const int c = threadIdx.x % C;
@@ -110,36 +110,36 @@ __global__ void __launch_bounds__(ColCfg<N>::T * C)
 
 // ---------------------------------------------------------------- host launchers
 
-template <int N>
+template <int N, int E>
 struct FarFieldColsRun {
   static ptx_status run(const BaseFarFieldCols& op, int64_t frames, hipStream_t s) {
-    launch_cols<N, -1>(op, frames, op.width, s);
+    launch_cols<N, E, -1>(op, frames, op.width, s);
     return check_launch("base far-field cols", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct GammaColsRun {
   static ptx_status run(const BaseGammaCols& op, int64_t frames, hipStream_t s) {
-    launch_cols<N, +1>(op, frames, op.width, s);
+    launch_cols<N, E, +1>(op, frames, op.width, s);
     return check_launch("base gamma cols", s);
   }
 };
 
 
-template <int N>
+template <int N, int E>
 struct StepColsRun {
   static ptx_status run(float2* tb, const float* measured, float* amp, float* loss_part, int64_t frames, int64_t width,
                         float gscale, hipStream_t s) {
-    using Cf = ColCfg<N>;
+    using Cf = ColCfg<N, E>;
     if (width >= Cf::CMAX) {
       constexpr int C = Cf::CMAX;
       const int64_t tiles = width / C;
-      k_base_step_cols<N, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(tb, measured, amp, loss_part,
+      k_base_step_cols<N, E, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(tb, measured, amp, loss_part,
                                                                                             width, tiles, gscale);
     } else {
       constexpr int C = 8;
       const int64_t tiles = width / C;
-      k_base_step_cols<N, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(tb, measured, amp, loss_part,
+      k_base_step_cols<N, E, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(tb, measured, amp, loss_part,
                                                                                             width, tiles, gscale);
     }
     return check_launch("base step cols", s);
@@ -165,16 +165,16 @@ ptx_status ptx_base_forward(const void* sample, int64_t H, int64_t W, const void
   hipStream_t s = (hipStream_t)stream;
   timing_begin(s);
   ScanGatherRows<true> r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, pos, (float2*)far_field};
-  st = dispatch_n<GatherRowsRun>(w, r1, B * h, s);
+  st = dispatch_ne<GatherRowsRun>(PASS_ROW, w, r1, B * h, s);
   if (st != PTX_OK) return st;
   BaseFarFieldCols c1{(float2*)far_field, amplitude, w};
-  return dispatch_n<FarFieldColsRun>(h, c1, B, s);
+  return dispatch_ne<FarFieldColsRun>(PASS_COL, h, c1, B, s);
 }
 
 size_t ptx_base_backward_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
   if (check_shapes("ptx_base_backward_workspace", H, W, h, w, B) != PTX_OK) return 0;
   ScatterPlan sp{};
-  dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
+  dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
   return carve_scan(nullptr, H, W, h, w, B, flags, false, sp, 0).bytes;
 }
 
@@ -200,7 +200,7 @@ ptx_status ptx_base_backward(const float* grad_output, const void* far_field, co
     return PTX_ERR_INVALID_ARGUMENT;
   }
   ScatterPlan sp{};
-  dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
+  dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
   const ScanWorkspace ws = carve_scan(workspace, H, W, h, w, B, flags, false, sp, 0);
   if (!workspace || ws_bytes < ws.bytes) {
     set_error("ptx_base_backward: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
@@ -208,7 +208,7 @@ ptx_status ptx_base_backward(const float* grad_output, const void* far_field, co
   }
   timing_begin(s);
   BaseGammaCols c2{(const float2*)far_field, grad_output, ws.tb, w};
-  st = dispatch_n<GammaColsRun>(h, c2, B, s);
+  st = dispatch_ne<GammaColsRun>(PASS_COL, h, c2, B, s);
   if (st != PTX_OK) return st;
   return scan_scatter<true>(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, pos, B,
                             (float2*)grad_sample, (float2*)grad_probe, flags, 1.0f, s);
@@ -217,9 +217,9 @@ ptx_status ptx_base_backward(const float* grad_output, const void* far_field, co
 size_t ptx_base_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
   if (check_shapes("ptx_base_step_workspace", H, W, h, w, B) != PTX_OK) return 0;
   ScatterPlan sp{};
-  dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
+  dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
   int64_t tiles = 0;
-  dispatch_n<ColTilesQuery>(h, w, &tiles);
+  dispatch_ne<ColTilesQuery>(PASS_COL, h, w, &tiles);
   return carve_scan(nullptr, H, W, h, w, B, flags, true, sp, tiles).bytes;
 }
 
@@ -246,9 +246,9 @@ ptx_status ptx_base_step(const void* sample, int64_t H, int64_t W, const void* p
     return PTX_ERR_INVALID_ARGUMENT;
   }
   ScatterPlan sp{};
-  dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
+  dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
   int64_t tiles = 0;
-  dispatch_n<ColTilesQuery>(h, w, &tiles);
+  dispatch_ne<ColTilesQuery>(PASS_COL, h, w, &tiles);
   const ScanWorkspace ws = carve_scan(workspace, H, W, h, w, B, flags, true, sp, tiles);
   if (!workspace || ws_bytes < ws.bytes) {
     set_error("ptx_base_step: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
@@ -256,9 +256,9 @@ ptx_status ptx_base_step(const void* sample, int64_t H, int64_t W, const void* p
   }
   timing_begin(s);
   ScanGatherRows<true> r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, pos, ws.tb};
-  st = dispatch_n<GatherRowsRun>(w, r1, B * h, s);
+  st = dispatch_ne<GatherRowsRun>(PASS_ROW, w, r1, B * h, s);
   if (st != PTX_OK) return st;
-  st = dispatch_n<StepColsRun>(h, ws.tb, measured, amplitude, ws.loss_part, B, w, 2.0f * loss_scale, s);
+  st = dispatch_ne<StepColsRun>(PASS_COL, h, ws.tb, measured, amplitude, ws.loss_part, B, w, 2.0f * loss_scale, s);
   if (st != PTX_OK) return st;
   loss_rows(ws.loss_part, tiles, B, 1.0f / (float)(h * w), loss_per_position, s);
   if ((st = check_launch("loss rows", s)) != PTX_OK) return st;

@@ -83,12 +83,12 @@ struct CodedSpecCols {
 };
 
 // P2: row iFFT -> X = v/N (stored), * C, row FFT (in place)
-template <int N>
+template <int N, int E>
 __global__ void __launch_bounds__(256)
     k_coded_x_rows(float2* __restrict__ tb, float2* __restrict__ xfwd, const float2* __restrict__ mask, int64_t H,
                    int64_t nrows, float inv_n) {
-  using Cf = RowCfg<N>;
-  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  using Cf = RowCfg<N, E>;
+  constexpr int T = Cf::T, S = Cf::S;
   __shared__ float2 lds[S * lds_len<N>()];
   const int seq = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -208,12 +208,12 @@ struct CodedGradZRows {
 
 // P4 + Q1 fused for the ds == 1 step: row iFFT -> Z, A = |Z|, loss partial per
 // row, gZ = Z gscale (A - Y)/max(A, eps), row FFT (in place)
-template <int N>
+template <int N, int E>
 __global__ void __launch_bounds__(256)
     k_coded_step_rows(float2* __restrict__ tb, const float* __restrict__ measured, float* __restrict__ amp_out,
                       float* __restrict__ loss_part, int64_t nrows, float gscale, float inv_n) {
-  using Cf = RowCfg<N>;
-  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  using Cf = RowCfg<N, E>;
+  constexpr int T = Cf::T, S = Cf::S;
   __shared__ float2 lds[S * lds_len<N>()];
   __shared__ float red[256];
   const int seq = threadIdx.x / T;
@@ -251,12 +251,12 @@ __global__ void __launch_bounds__(256)
 
 // Q3: per (row group, chunk of views): row iFFT -> gY = v/N; mask partial += gY conj(X_b);
 // gX = gY conj(C); row FFT (in place)
-template <int N>
+template <int N, int E>
 __global__ void __launch_bounds__(256)
     k_coded_y_rows(float2* __restrict__ tb, const float2* __restrict__ xfwd, const float2* __restrict__ mask,
                    int64_t H, int64_t B, int chunk, int ngroups, float inv_n, float2* __restrict__ gc_part) {
-  using Cf = RowCfg<N>;
-  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  using Cf = RowCfg<N, E>;
+  constexpr int T = Cf::T, S = Cf::S;
   __shared__ float2 lds[S * lds_len<N>()];
   const int seq = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -299,13 +299,13 @@ __global__ void __launch_bounds__(256)
 }
 
 // Q4: per (column tile, chunk of views): column FFT; spectrum partial += conj(H1 R_b) v / N
-template <int N, int C>
-__global__ void __launch_bounds__(ColCfg<N>::T * C)
+template <int N, int E, int C>
+__global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     k_coded_spec_cols(const float2* __restrict__ tb, const float2* __restrict__ h1, const float2* __restrict__ ex,
                       const float2* __restrict__ ey, int64_t B, int64_t width, int64_t tiles, int chunk, float inv_n,
                       float2* __restrict__ gs_part) {
-  using Cf = ColCfg<N>;
-  constexpr int E = Cf::E, T = Cf::T;
+  using Cf = ColCfg<N, E>;
+  constexpr int T = Cf::T;
   __shared__ float2 lds[C * lds_len<N>()];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
@@ -342,49 +342,49 @@ __global__ void __launch_bounds__(ColCfg<N>::T * C)
 
 // ---------------------------------------------------------------- host launchers
 
-template <int N>
+template <int N, int E>
 struct SpecColsRun {
   static ptx_status run(const CodedSpecCols& op, int64_t frames, hipStream_t s) {
-    launch_cols<N, +1>(op, frames, op.width, s);
+    launch_cols<N, E, +1>(op, frames, op.width, s);
     return check_launch("coded spectrum cols", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct XRowsRun {
   static ptx_status run(float2* tb, float2* xfwd, const float2* mask, int64_t H, int64_t nrows, float inv_n,
                         hipStream_t s) {
-    const int64_t blocks = (nrows + RowCfg<N>::S - 1) / RowCfg<N>::S;
-    k_coded_x_rows<N><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(tb, xfwd, mask, H, nrows, inv_n);
+    const int64_t blocks = (nrows + RowCfg<N, E>::S - 1) / RowCfg<N, E>::S;
+    k_coded_x_rows<N, E><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(tb, xfwd, mask, H, nrows, inv_n);
     return check_launch("coded x rows", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct TransferRun {
   static ptx_status run(float2* tb, const float2* tr, int64_t frames, int64_t width, bool conj, hipStream_t s) {
-    launch_transfer_cols<N>(tb, tr, frames, width, conj, s);
+    launch_transfer_cols<N, E>(tb, tr, frames, width, conj, s);
     return check_launch(conj ? "coded transfer* cols" : "coded transfer cols", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct ZRowsRun {
   static ptx_status run(const CodedZRows& op, int64_t nrows, hipStream_t s) {
-    launch_rows<N, +1>(op, nrows, s);
+    launch_rows<N, E, +1>(op, nrows, s);
     return check_launch("coded z rows", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct GradZRowsRun {
   static ptx_status run(const CodedGradZRows& op, int64_t nrows, hipStream_t s) {
-    launch_rows<N, -1>(op, nrows, s);
+    launch_rows<N, E, -1>(op, nrows, s);
     return check_launch("coded grad-z rows", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct StepRowsRun {
   static ptx_status run(float2* tb, const float* measured, float* amp, float* loss_part, int64_t nrows, float gscale,
                         float inv_n, hipStream_t s) {
-    const int64_t blocks = (nrows + RowCfg<N>::S - 1) / RowCfg<N>::S;
-    k_coded_step_rows<N><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(tb, measured, amp, loss_part, nrows, gscale,
+    const int64_t blocks = (nrows + RowCfg<N, E>::S - 1) / RowCfg<N, E>::S;
+    k_coded_step_rows<N, E><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(tb, measured, amp, loss_part, nrows, gscale,
                                                                       inv_n);
     return check_launch("coded step rows", s);
   }
@@ -395,10 +395,10 @@ struct Chunks {
   int col_tiles, col_c, col_chunk, col_nchunks;  // Q4
 };
 
-template <int N>
+template <int N, int E>
 struct RowChunkQuery {
   static ptx_status run(int64_t H, int64_t B, Chunks* c) {
-    c->row_groups = (int)((H + RowCfg<N>::S - 1) / RowCfg<N>::S);
+    c->row_groups = (int)((H + RowCfg<N, E>::S - 1) / RowCfg<N, E>::S);
     int64_t want = (2048 + c->row_groups - 1) / c->row_groups;
     want = want < 1 ? 1 : (want > B ? B : want);
     c->row_chunk = (int)((B + want - 1) / want);
@@ -406,10 +406,10 @@ struct RowChunkQuery {
     return PTX_OK;
   }
 };
-template <int N>
+template <int N, int E>
 struct ColChunkQuery {
   static ptx_status run(int64_t W, int64_t B, Chunks* c) {
-    c->col_c = W >= ColCfg<N>::CMAX ? ColCfg<N>::CMAX : 8;
+    c->col_c = W >= ColCfg<N, E>::CMAX ? ColCfg<N, E>::CMAX : 8;
     c->col_tiles = (int)(W / c->col_c);
     int64_t want = (2048 + c->col_tiles - 1) / c->col_tiles;
     want = want < 1 ? 1 : (want > B ? B : want);
@@ -418,39 +418,39 @@ struct ColChunkQuery {
     return PTX_OK;
   }
 };
-template <int N>
+template <int N, int E>
 struct YRowsRun {
   static ptx_status run(float2* tb, const float2* xfwd, const float2* mask, int64_t H, int64_t B, const Chunks& c,
                         float inv_n, float2* gc_part, hipStream_t s) {
-    k_coded_y_rows<N><<<dim3((unsigned)(c.row_groups * c.row_nchunks)), dim3(256), 0, s>>>(
+    k_coded_y_rows<N, E><<<dim3((unsigned)(c.row_groups * c.row_nchunks)), dim3(256), 0, s>>>(
         tb, xfwd, mask, H, B, c.row_chunk, c.row_groups, inv_n, gc_part);
     return check_launch("coded y rows", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct SpecGradColsRun {
   static ptx_status run(const float2* tb, const float2* h1, const float2* ex, const float2* ey, int64_t B, int64_t W,
                         const Chunks& c, float inv_n, float2* gs_part, hipStream_t s) {
     const dim3 grid((unsigned)(c.col_tiles * c.col_nchunks));
-    if (c.col_c == ColCfg<N>::CMAX) {
-      constexpr int C = ColCfg<N>::CMAX;
-      k_coded_spec_cols<N, C><<<grid, dim3(ColCfg<N>::T * C), 0, s>>>(tb, h1, ex, ey, B, W, c.col_tiles, c.col_chunk,
+    if (c.col_c == ColCfg<N, E>::CMAX) {
+      constexpr int C = ColCfg<N, E>::CMAX;
+      k_coded_spec_cols<N, E, C><<<grid, dim3(ColCfg<N, E>::T * C), 0, s>>>(tb, h1, ex, ey, B, W, c.col_tiles, c.col_chunk,
                                                                        inv_n, gs_part);
     } else {
-      k_coded_spec_cols<N, 8><<<grid, dim3(ColCfg<N>::T * 8), 0, s>>>(tb, h1, ex, ey, B, W, c.col_tiles, c.col_chunk,
+      k_coded_spec_cols<N, E, 8><<<grid, dim3(ColCfg<N, E>::T * 8), 0, s>>>(tb, h1, ex, ey, B, W, c.col_tiles, c.col_chunk,
                                                                        inv_n, gs_part);
     }
     return check_launch("coded spectrum-grad cols", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct RowsOOPRun {
   static ptx_status run(const float2* in, float2* out, int64_t nrows, int dir, hipStream_t s) {
     RowsOutOfPlace op{in, out, 1.0f};
     if (dir < 0)
-      launch_rows<N, -1>(op, nrows, s);
+      launch_rows<N, E, -1>(op, nrows, s);
     else
-      launch_rows<N, +1>(op, nrows, s);
+      launch_rows<N, E, +1>(op, nrows, s);
     return check_launch(dir < 0 ? "fft rows" : "ifft rows", s);
   }
 };
@@ -476,13 +476,13 @@ struct ColsInto {
     }
   }
 };
-template <int N>
+template <int N, int E>
 struct ColsIntoRun {
   static ptx_status run(const ColsInto& op, int dir, hipStream_t s) {
     if (dir < 0)
-      launch_cols<N, -1>(op, 1, op.width, s);
+      launch_cols<N, E, -1>(op, 1, op.width, s);
     else
-      launch_cols<N, +1>(op, 1, op.width, s);
+      launch_cols<N, E, +1>(op, 1, op.width, s);
     return check_launch(dir < 0 ? "fft cols" : "ifft cols", s);
   }
 };
@@ -490,10 +490,10 @@ struct ColsIntoRun {
 ptx_status fft2_frame(const float2* in, float2* out, int64_t H, int64_t W, int dir, int accumulate, float2* scratch,
                       hipStream_t s) {
   // rows out of place into scratch, columns from scratch into out
-  ptx_status st = dispatch_n<RowsOOPRun>(W, in, scratch, H, dir, s);
+  ptx_status st = dispatch_ne<RowsOOPRun>(PASS_ROW, W, in, scratch, H, dir, s);
   if (st != PTX_OK) return st;
   ColsInto c{scratch, out, W, accumulate};
-  return dispatch_n<ColsIntoRun>(H, c, dir, s);
+  return dispatch_ne<ColsIntoRun>(PASS_COL, H, c, dir, s);
 }
 
 struct CodedWorkspace {
@@ -547,8 +547,8 @@ ptx_status check(const char* fn, int64_t H, int64_t W, int64_t B, int64_t ds) {
 
 Chunks plan(int64_t H, int64_t W, int64_t B) {
   Chunks c{};
-  dispatch_n<RowChunkQuery>(W, H, B, &c);
-  dispatch_n<ColChunkQuery>(H, W, B, &c);
+  dispatch_ne<RowChunkQuery>(PASS_ROW_LOOP, W, H, B, &c);
+  dispatch_ne<ColChunkQuery>(PASS_COL_LOOP, H, W, B, &c);
   return c;
 }
 
@@ -563,10 +563,10 @@ ptx_status forward_to_t(const float2* spec, const float2* mask, const float2* t1
                         int64_t W, int64_t B, const CodedWorkspace& ws, float2* xfwd, hipStream_t s) {
   const float inv_n = 1.0f / (float)(H * W);
   CodedSpecCols p1{spec, t1, ws.ex, ws.ey, ws.tb, H, W};
-  ptx_status st = dispatch_n<SpecColsRun>(H, p1, B, s);
+  ptx_status st = dispatch_ne<SpecColsRun>(PASS_COL, H, p1, B, s);
   if (st != PTX_OK) return st;
-  if ((st = dispatch_n<XRowsRun>(W, ws.tb, xfwd, mask, H, B * H, inv_n, s)) != PTX_OK) return st;
-  return dispatch_n<TransferRun>(H, ws.tb, t2, B, W, false, s);
+  if ((st = dispatch_ne<XRowsRun>(PASS_ROW, W, ws.tb, xfwd, mask, H, B * H, inv_n, s)) != PTX_OK) return st;
+  return dispatch_ne<TransferRun>(PASS_COL, H, ws.tb, t2, B, W, false, s);
 }
 
 // Q2..end: T holds row-FFT of gZ; produces grad_object and grad_coded
@@ -574,10 +574,10 @@ ptx_status backward_from_t(const float2* xfwd, const float2* mask, const float2*
                            int64_t W, int64_t B, const Chunks& c, const CodedWorkspace& ws, float2* gobj,
                            float2* gcoded, int accumulate, hipStream_t s) {
   const float inv_n = 1.0f / (float)(H * W);
-  ptx_status st = dispatch_n<TransferRun>(H, ws.tb, t2, B, W, true, s);
+  ptx_status st = dispatch_ne<TransferRun>(PASS_COL, H, ws.tb, t2, B, W, true, s);
   if (st != PTX_OK) return st;
-  if ((st = dispatch_n<YRowsRun>(W, ws.tb, xfwd, mask, H, B, c, inv_n, ws.gc_part, s)) != PTX_OK) return st;
-  if ((st = dispatch_n<SpecGradColsRun>(H, ws.tb, t1, ws.ex, ws.ey, B, W, c, inv_n, ws.gs_part, s)) != PTX_OK)
+  if ((st = dispatch_ne<YRowsRun>(PASS_ROW_LOOP, W, ws.tb, xfwd, mask, H, B, c, inv_n, ws.gc_part, s)) != PTX_OK) return st;
+  if ((st = dispatch_ne<SpecGradColsRun>(PASS_COL_LOOP, H, ws.tb, t1, ws.ex, ws.ey, B, W, c, inv_n, ws.gs_part, s)) != PTX_OK)
     return st;
   sum_parts2(ws.gc_part, c.row_nchunks, H * W, gcoded, accumulate, s);
   if ((st = check_launch("mask partial sum", s)) != PTX_OK) return st;
@@ -621,7 +621,7 @@ ptx_status ptx_coded_forward(const void* object_field, int64_t H, int64_t W, con
                          W, B, ws, (float2*)x_forward, s)) != PTX_OK)
     return st;
   CodedZRows p4{ws.tb, (float2*)sensor_field, downsample == 1 ? amplitude : nullptr, 1.0f / (float)(H * W)};
-  if ((st = dispatch_n<ZRowsRun>(W, p4, B * H, s)) != PTX_OK) return st;
+  if ((st = dispatch_ne<ZRowsRun>(PASS_ROW, W, p4, B * H, s)) != PTX_OK) return st;
   if (downsample > 1) {
     const int64_t n = B * (H / downsample) * (W / downsample);
     k_coded_pool<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(
@@ -679,7 +679,7 @@ ptx_status ptx_coded_backward(const float* grad_output, const void* object_spect
     scmap = ws.sc;
   }
   CodedGradZRows q1{(const float2*)sensor_field, scmap, ws.tb, H, (int)downsample};
-  if ((st = dispatch_n<GradZRowsRun>(W, q1, B * H, s)) != PTX_OK) return st;
+  if ((st = dispatch_ne<GradZRowsRun>(PASS_ROW, W, q1, B * H, s)) != PTX_OK) return st;
   return backward_from_t((const float2*)x_forward, (const float2*)coded_surface, (const float2*)transfer_1,
                          (const float2*)transfer_2, H, W, B, c, ws, (float2*)grad_object, (float2*)grad_coded, acc, s);
 }
@@ -725,7 +725,7 @@ ptx_status ptx_coded_step(const void* object_field, int64_t H, int64_t W, const 
     return st;
   const float inv_n = 1.0f / (float)(H * W);
   if (ds == 1) {
-    if ((st = dispatch_n<StepRowsRun>(W, ws.tb, measured, amplitude, ws.loss_part, B * H, 2.0f * loss_scale, inv_n,
+    if ((st = dispatch_ne<StepRowsRun>(PASS_ROW, W, ws.tb, measured, amplitude, ws.loss_part, B * H, 2.0f * loss_scale, inv_n,
                                       s)) != PTX_OK)
       return st;
     loss_rows(ws.loss_part, H, B, inv_n, loss_per_position, s);
@@ -733,7 +733,7 @@ ptx_status ptx_coded_step(const void* object_field, int64_t H, int64_t W, const 
   } else {
     // Z in place in T, then pooled amplitude / loss / scale map, then gZ row FFT
     CodedZRows p4{ws.tb, ws.tb, nullptr, inv_n};
-    if ((st = dispatch_n<ZRowsRun>(W, p4, B * H, s)) != PTX_OK) return st;
+    if ((st = dispatch_ne<ZRowsRun>(PASS_ROW, W, p4, B * H, s)) != PTX_OK) return st;
     const int64_t n = B * lh * lw;
     k_coded_pool<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(ws.tb, B, H, W, (int)ds, measured, amplitude,
                                                                         ws.sc, ws.loss_part, 2.0f * loss_scale, 2);
@@ -741,7 +741,7 @@ ptx_status ptx_coded_step(const void* object_field, int64_t H, int64_t W, const 
     loss_rows(ws.loss_part, lh * lw, B, 1.0f / (float)(lh * lw), loss_per_position, s);
     if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
     CodedGradZRows q1{ws.tb, ws.sc, ws.tb, H, (int)ds};
-    if ((st = dispatch_n<GradZRowsRun>(W, q1, B * H, s)) != PTX_OK) return st;
+    if ((st = dispatch_ne<GradZRowsRun>(PASS_ROW, W, q1, B * H, s)) != PTX_OK) return st;
   }
   return backward_from_t(ws.xfwd, (const float2*)coded_surface, (const float2*)transfer_1, (const float2*)transfer_2,
                          H, W, B, c, ws, (float2*)grad_object, (float2*)grad_coded, acc, s);

@@ -1,4 +1,5 @@
 // Error reporting, version queries and the standalone batched 2-D FFT entry point.
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 
@@ -14,6 +15,22 @@ void set_error(const char* fmt, ...) {
   va_start(ap, fmt);
   vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
   va_end(ap);
+}
+
+// ---------------------------------------------------------------- tuning table
+static std::atomic<int> g_tune[PASS_KINDS][16];  // 0 = built-in default
+
+static int default_elems(int kind, int64_t n) {
+  if (n <= 8) return 8;
+  if (n == 16) return 16;
+  return kind == PASS_ROW_LOOP ? 16 : 8;
+}
+
+int tuned_elems(int kind, int64_t n) {
+  if (kind < 0 || kind >= PASS_KINDS || !is_pow2(n)) return default_elems(kind, n);
+  const int l = ilog2((int)n);
+  const int v = (l >= 0 && l < 16) ? g_tune[kind][l].load(std::memory_order_relaxed) : 0;
+  return v ? v : default_elems(kind, n);
 }
 
 static thread_local hipEvent_t* t_events = nullptr;
@@ -43,26 +60,26 @@ ptx_status check_launch(const char* what, hipStream_t stream) {
   return PTX_OK;
 }
 
-template <int N>
+template <int N, int E>
 struct FftRowsRun {
   static ptx_status run(float2* data, int64_t nrows, int inverse, hipStream_t s) {
     RowsInPlace op{data, 1.0f};
     if (inverse)
-      launch_rows<N, +1>(op, nrows, s);
+      launch_rows<N, E, +1>(op, nrows, s);
     else
-      launch_rows<N, -1>(op, nrows, s);
+      launch_rows<N, E, -1>(op, nrows, s);
     return check_launch("fft2d rows", s);
   }
 };
 
-template <int N>
+template <int N, int E>
 struct FftColsRun {
   static ptx_status run(float2* data, int64_t frames, int64_t width, int inverse, hipStream_t s) {
     ColsInPlace op{data, width, 1.0f};
     if (inverse)
-      launch_cols<N, +1>(op, frames, width, s);
+      launch_cols<N, E, +1>(op, frames, width, s);
     else
-      launch_cols<N, -1>(op, frames, width, s);
+      launch_cols<N, E, -1>(op, frames, width, s);
     return check_launch("fft2d cols", s);
   }
 };
@@ -88,6 +105,29 @@ ptx_status ptx_timing_events(void* const* events, int n) {
 
 int ptx_timing_count(void) { return t_used; }
 
+ptx_status ptx_tune_set(int kind, int64_t n, int elems) {
+  if (kind < 0 || kind >= PASS_KINDS) {
+    set_error("ptx_tune_set: pass kind %d out of range [0, %d)", kind, (int)PASS_KINDS);
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  if (!size_ok(n)) {
+    set_error("ptx_tune_set: length %lld unsupported", (long long)n);
+    return PTX_ERR_UNSUPPORTED;
+  }
+  if (elems != 0 && elems != 8 && elems != 16) {
+    set_error("ptx_tune_set: elements per thread must be 8 or 16 (0 = default), got %d", elems);
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  if (n == 8 && elems == 16) {
+    set_error("ptx_tune_set: length 8 takes 8 elements per thread");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  g_tune[kind][ilog2((int)n)].store(elems, std::memory_order_relaxed);
+  return PTX_OK;
+}
+
+int ptx_tune_get(int kind, int64_t n) { return size_ok(n) ? tuned_elems(kind, n) : 0; }
+
 const char* ptx_timing_label(int k) { return (k >= 0 && k < t_used) ? t_labels[k] : ""; }
 
 const char* ptx_version(void) { return "ptychox_hip 0.1.0 (gfx950)"; }
@@ -109,9 +149,9 @@ ptx_status ptx_fft2d(void* data, int64_t batch, int64_t h, int64_t w, int invers
   hipStream_t s = (hipStream_t)stream;
   timing_begin(s);
   float2* d = (float2*)data;
-  ptx_status st = dispatch_n<FftRowsRun>(w, d, batch * h, inverse, s);
+  ptx_status st = dispatch_ne<FftRowsRun>(PASS_ROW, w, d, batch * h, inverse, s);
   if (st != PTX_OK) return st;
-  return dispatch_n<FftColsRun>(h, d, batch, w, inverse, s);
+  return dispatch_ne<FftColsRun>(PASS_COL, h, d, batch, w, inverse, s);
 }
 
 }  // extern "C"

@@ -118,12 +118,12 @@ struct FpmGammaCols {
 
 // ---------------------------------------------------------------- C12: fused column pass of the step
 // column iFFT -> A = |v|/(hw), loss, Gamma*sign = G v/(hw)/max(A, eps) (the two signs cancel) -> column FFT
-template <int N, int C>
-__global__ void __launch_bounds__(ColCfg<N>::T * C)
+template <int N, int E, int C>
+__global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     k_fpm_step_cols(float2* __restrict__ tb, const float* __restrict__ measured, float* __restrict__ amp_out,
                     float* __restrict__ loss_part, int64_t width, int64_t tiles_per_frame, float gscale, float inv_n) {
-  using Cf = ColCfg<N>;
-  constexpr int E = Cf::E, T = Cf::T;
+  using Cf = ColCfg<N, E>;
+  constexpr int T = Cf::T;
   __shared__ float2 lds[C * lds_len<N>()];
   __shared__ float red[16];
   const int c = threadIdx.x % C;
@@ -154,14 +154,14 @@ __global__ void __launch_bounds__(ColCfg<N>::T * C)
 }
 
 // ---------------------------------------------------------------- R2: row FFT, pupil partials, spectrum scatter
-template <int N>
+template <int N, int E>
 __global__ void __launch_bounds__(256)
     k_fpm_scatter_rows(const float2* __restrict__ tb, const float2* __restrict__ spec, int64_t H, int64_t W,
                        const float2* __restrict__ pupil, int h, const int64_t* __restrict__ led, int64_t B, int chunk,
                        int ngroups, float inv_n, float2* __restrict__ gspec, float2* __restrict__ contrib,
                        float2* __restrict__ gq_part) {
-  using Cf = RowCfg<N>;
-  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  using Cf = RowCfg<N, E>;
+  constexpr int T = Cf::T, S = Cf::S;
   __shared__ float2 lds[S * lds_len<N>()];
   const int seq = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -237,76 +237,76 @@ struct ColsToOut {
 
 // ---------------------------------------------------------------- host launchers
 
-template <int N>
+template <int N, int E>
 struct RowsRun {
   template <int DIR, class Op>
   static ptx_status go(const Op& op, int64_t nrows, hipStream_t s, const char* what) {
-    launch_rows<N, DIR>(op, nrows, s);
+    launch_rows<N, E, DIR>(op, nrows, s);
     return check_launch(what, s);
   }
 };
-template <int N>
+template <int N, int E>
 struct ObjFftRows {
   static ptx_status run(const float2* in, float2* out, int64_t nrows, int dir, hipStream_t s) {
     RowsOutOfPlace op{in, out, 1.0f};
-    return dir < 0 ? RowsRun<N>::template go<-1>(op, nrows, s, "fft rows")
-                   : RowsRun<N>::template go<+1>(op, nrows, s, "ifft rows");
+    return dir < 0 ? RowsRun<N, E>::template go<-1>(op, nrows, s, "fft rows")
+                   : RowsRun<N, E>::template go<+1>(op, nrows, s, "ifft rows");
   }
 };
-template <int N>
+template <int N, int E>
 struct ObjFftColsOut {
   static ptx_status run(const ColsToOut& op, int dir, hipStream_t s) {
     if (dir < 0)
-      launch_cols<N, -1>(op, 1, op.width, s);
+      launch_cols<N, E, -1>(op, 1, op.width, s);
     else
-      launch_cols<N, +1>(op, 1, op.width, s);
+      launch_cols<N, E, +1>(op, 1, op.width, s);
     return check_launch(dir < 0 ? "fft cols" : "ifft cols", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct FpmGatherRun {
   static ptx_status run(const FpmGatherRows& op, int64_t nrows, hipStream_t s) {
-    launch_rows<N, +1>(op, nrows, s);
+    launch_rows<N, E, +1>(op, nrows, s);
     return check_launch("fpm gather rows", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct FpmImageColsRun {
   static ptx_status run(const FpmImageCols& op, int64_t frames, hipStream_t s) {
-    launch_cols<N, +1>(op, frames, op.width, s);
+    launch_cols<N, E, +1>(op, frames, op.width, s);
     return check_launch("fpm image cols", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct FpmGammaColsRun {
   static ptx_status run(const FpmGammaCols& op, int64_t frames, hipStream_t s) {
-    launch_cols<N, -1>(op, frames, op.width, s);
+    launch_cols<N, E, -1>(op, frames, op.width, s);
     return check_launch("fpm gamma cols", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct FpmStepColsRun {
   static ptx_status run(float2* tb, const float* measured, float* amp, float* loss_part, int64_t frames, int64_t width,
                         float gscale, float inv_n, hipStream_t s) {
-    using Cf = ColCfg<N>;
+    using Cf = ColCfg<N, E>;
     if (width >= Cf::CMAX) {
       constexpr int C = Cf::CMAX;
       const int64_t tiles = width / C;
-      k_fpm_step_cols<N, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(
+      k_fpm_step_cols<N, E, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(
           tb, measured, amp, loss_part, width, tiles, gscale, inv_n);
     } else {
       constexpr int C = 8;
       const int64_t tiles = width / C;
-      k_fpm_step_cols<N, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(
+      k_fpm_step_cols<N, E, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(
           tb, measured, amp, loss_part, width, tiles, gscale, inv_n);
     }
     return check_launch("fpm step cols", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct ColTiles {
   static ptx_status run(int64_t width, int64_t* tiles) {
-    *tiles = width >= ColCfg<N>::CMAX ? width / ColCfg<N>::CMAX : width / 8;
+    *tiles = width >= ColCfg<N, E>::CMAX ? width / ColCfg<N, E>::CMAX : width / 8;
     return PTX_OK;
   }
 };
@@ -314,10 +314,10 @@ struct ColTiles {
 struct Plan {
   int S, ngroups, chunk, nchunks;
 };
-template <int N>
+template <int N, int E>
 struct PlanQuery {
   static ptx_status run(int64_t h, int64_t B, Plan* p) {
-    p->S = RowCfg<N>::S;
+    p->S = RowCfg<N, E>::S;
     p->ngroups = (int)((h + p->S - 1) / p->S);
     int64_t want = (1024 + p->ngroups - 1) / p->ngroups;
     if (want > B) want = B > 0 ? B : 1;
@@ -329,12 +329,12 @@ struct PlanQuery {
     return PTX_OK;
   }
 };
-template <int N>
+template <int N, int E>
 struct ScatterRun {
   static ptx_status run(const float2* tb, const float2* spec, int64_t H, int64_t W, const float2* pupil, int h,
                         const int64_t* led, int64_t B, const Plan& p, float inv_n, float2* gspec, float2* contrib,
                         float2* gq_part, hipStream_t s) {
-    k_fpm_scatter_rows<N><<<dim3((unsigned)(p.ngroups * p.nchunks)), dim3(256), 0, s>>>(
+    k_fpm_scatter_rows<N, E><<<dim3((unsigned)(p.ngroups * p.nchunks)), dim3(256), 0, s>>>(
         tb, spec, H, W, pupil, h, led, B, p.chunk, p.ngroups, inv_n, gspec, contrib, gq_part);
     return check_launch("fpm scatter rows", s);
   }
@@ -389,10 +389,10 @@ ptx_status check(const char* fn, int64_t H, int64_t W, int64_t h, int64_t w, int
 
 // object spectrum: rows (out of place) then columns into `spec`
 ptx_status spectrum_of(const float2* obj, float2* spec, int64_t H, int64_t W, hipStream_t s) {
-  ptx_status st = dispatch_n<ObjFftRows>(W, obj, spec, H, -1, s);
+  ptx_status st = dispatch_ne<ObjFftRows>(PASS_ROW, W, obj, spec, H, -1, s);
   if (st != PTX_OK) return st;
   ColsToOut c{spec, spec, W, 0};
-  return dispatch_n<ObjFftColsOut>(H, c, -1, s);
+  return dispatch_ne<ObjFftColsOut>(PASS_COL, H, c, -1, s);
 }
 
 // R2 + pupil reduction + spectrum scatter + grad_O = HW ifft2(grad_O^)
@@ -407,7 +407,7 @@ ptx_status fpm_tail(const FpmWorkspace& ws, const Plan& p, const float2* spec, i
     if ((st = check_launch("zero fill", s)) != PTX_OK) return st;
   }
   const float inv_n = 1.0f / (float)(h * w);
-  st = dispatch_n<ScatterRun>(w, ws.tb, spec, H, W, pupil, (int)h, led, B, p, inv_n, ws.gspec, det ? ws.tb : nullptr,
+  st = dispatch_ne<ScatterRun>(PASS_ROW_LOOP, w, ws.tb, spec, H, W, pupil, (int)h, led, B, p, inv_n, ws.gspec, det ? ws.tb : nullptr,
                               ws.gq_part, s);
   if (st != PTX_OK) return st;
   sum_parts2(ws.gq_part, p.nchunks, h * w, gpupil, acc ? 1 : 0, s);
@@ -421,10 +421,10 @@ ptx_status fpm_tail(const FpmWorkspace& ws, const Plan& p, const float2* spec, i
                                                                     ws.lists, ws.counts, tg.tiles_x, ws.gspec, H, W, 0);
     if ((st = check_launch("tile gather", s)) != PTX_OK) return st;
   }
-  st = dispatch_n<ObjFftRows>(W, ws.gspec, ws.gspec, H, +1, s);
+  st = dispatch_ne<ObjFftRows>(PASS_ROW, W, ws.gspec, ws.gspec, H, +1, s);
   if (st != PTX_OK) return st;
   ColsToOut c{ws.gspec, gobj, W, acc ? 1 : 0};
-  return dispatch_n<ObjFftColsOut>(H, c, +1, s);
+  return dispatch_ne<ObjFftColsOut>(PASS_COL, H, c, +1, s);
 }
 
 }  // namespace
@@ -448,16 +448,16 @@ ptx_status ptx_fpm_forward(const void* object_field, int64_t H, int64_t W, const
   st = spectrum_of((const float2*)object_field, (float2*)object_fft, H, W, s);
   if (st != PTX_OK || B == 0) return st;
   FpmGatherRows r1{(const float2*)object_fft, H, W, (const float2*)pupil, (int)h, led, (float2*)image_field};
-  st = dispatch_n<FpmGatherRun>(w, r1, B * h, s);
+  st = dispatch_ne<FpmGatherRun>(PASS_ROW, w, r1, B * h, s);
   if (st != PTX_OK) return st;
   FpmImageCols c1{(float2*)image_field, amplitude, w, 1.0f / (float)(h * w)};
-  return dispatch_n<FpmImageColsRun>(h, c1, B, s);
+  return dispatch_ne<FpmImageColsRun>(PASS_COL, h, c1, B, s);
 }
 
 size_t ptx_fpm_backward_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
   if (check("ptx_fpm_backward_workspace", H, W, h, w, B) != PTX_OK) return 0;
   Plan p{};
-  dispatch_n<PlanQuery>(w, h, B, &p);
+  dispatch_ne<PlanQuery>(PASS_ROW_LOOP, w, h, B, &p);
   return carve(nullptr, H, W, h, w, B, flags, false, p, 0).bytes;
 }
 
@@ -485,14 +485,14 @@ ptx_status ptx_fpm_backward(const float* grad_output, const void* image_field, c
     return PTX_ERR_INVALID_ARGUMENT;
   }
   Plan p{};
-  dispatch_n<PlanQuery>(w, h, B, &p);
+  dispatch_ne<PlanQuery>(PASS_ROW_LOOP, w, h, B, &p);
   const FpmWorkspace ws = carve(workspace, H, W, h, w, B, flags, false, p, 0);
   if (!workspace || ws_bytes < ws.bytes) {
     set_error("ptx_fpm_backward: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
     return PTX_ERR_WORKSPACE;
   }
   FpmGammaCols c2{(const float2*)image_field, grad_output, ws.tb, w};
-  st = dispatch_n<FpmGammaColsRun>(h, c2, B, s);
+  st = dispatch_ne<FpmGammaColsRun>(PASS_COL, h, c2, B, s);
   if (st != PTX_OK) return st;
   return fpm_tail(ws, p, (const float2*)object_fft, H, W, (const float2*)pupil, h, w, led, B, (float2*)grad_object,
                   (float2*)grad_pupil, flags, s);
@@ -501,9 +501,9 @@ ptx_status ptx_fpm_backward(const float* grad_output, const void* image_field, c
 size_t ptx_fpm_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
   if (check("ptx_fpm_step_workspace", H, W, h, w, B) != PTX_OK) return 0;
   Plan p{};
-  dispatch_n<PlanQuery>(w, h, B, &p);
+  dispatch_ne<PlanQuery>(PASS_ROW_LOOP, w, h, B, &p);
   int64_t tiles = 0;
-  dispatch_n<ColTiles>(h, w, &tiles);
+  dispatch_ne<ColTiles>(PASS_COL, h, w, &tiles);
   return carve(nullptr, H, W, h, w, B, flags, true, p, tiles).bytes;
 }
 
@@ -531,9 +531,9 @@ ptx_status ptx_fpm_step(const void* object_field, int64_t H, int64_t W, const vo
     return PTX_ERR_INVALID_ARGUMENT;
   }
   Plan p{};
-  dispatch_n<PlanQuery>(w, h, B, &p);
+  dispatch_ne<PlanQuery>(PASS_ROW_LOOP, w, h, B, &p);
   int64_t tiles = 0;
-  dispatch_n<ColTiles>(h, w, &tiles);
+  dispatch_ne<ColTiles>(PASS_COL, h, w, &tiles);
   const FpmWorkspace ws = carve(workspace, H, W, h, w, B, flags, true, p, tiles);
   if (!workspace || ws_bytes < ws.bytes) {
     set_error("ptx_fpm_step: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
@@ -542,10 +542,10 @@ ptx_status ptx_fpm_step(const void* object_field, int64_t H, int64_t W, const vo
   st = spectrum_of((const float2*)object_field, ws.spec, H, W, s);
   if (st != PTX_OK) return st;
   FpmGatherRows r1{ws.spec, H, W, (const float2*)pupil, (int)h, led, ws.tb};
-  st = dispatch_n<FpmGatherRun>(w, r1, B * h, s);
+  st = dispatch_ne<FpmGatherRun>(PASS_ROW, w, r1, B * h, s);
   if (st != PTX_OK) return st;
   const float inv_n = 1.0f / (float)(h * w);
-  st = dispatch_n<FpmStepColsRun>(h, ws.tb, measured, amplitude, ws.loss_part, B, w, 2.0f * loss_scale, inv_n, s);
+  st = dispatch_ne<FpmStepColsRun>(PASS_COL, h, ws.tb, measured, amplitude, ws.loss_part, B, w, 2.0f * loss_scale, inv_n, s);
   if (st != PTX_OK) return st;
   loss_rows(ws.loss_part, tiles, B, inv_n, loss_per_position, s);
   if ((st = check_launch("loss rows", s)) != PTX_OK) return st;

@@ -32,19 +32,29 @@ ptx_status check_launch(const char* what, hipStream_t stream = nullptr);
 // Records timing event 0 when armed; called by each entry point before its first kernel.
 void timing_begin(hipStream_t stream);
 
-// Call F<N>::run(args...) for the runtime power-of-two n.
-template <template <int> class F, typename... Args>
-ptx_status dispatch_n(int64_t n, Args&&... args) {
+// Pass kinds with their own elements-per-thread setting in the tuning table.
+enum PassKind { PASS_ROW = 0, PASS_ROW_LOOP = 1, PASS_COL = 2, PASS_COL_LOOP = 3, PASS_KINDS = 4 };
+
+// Elements per thread (8 or 16) the tuning table holds for (kind, n); n a
+// supported power of two.  Defaults come from MI355X measurements (radix-8
+// passes run ~15 % faster than radix-16 for the plain row / column passes at
+// 256..1024 points, the view-looping row pass prefers radix 16).
+int tuned_elems(int kind, int64_t n);
+
+// Call F<N, E>::run(args...) for the runtime power-of-two n and the tuned E.
+template <template <int, int> class F, typename... Args>
+ptx_status dispatch_ne(int kind, int64_t n, Args&&... args) {
+  const bool e16 = tuned_elems(kind, n) == 16;
   switch (n) {
-    case 8: return F<8>::run(args...);
-    case 16: return F<16>::run(args...);
-    case 32: return F<32>::run(args...);
-    case 64: return F<64>::run(args...);
-    case 128: return F<128>::run(args...);
-    case 256: return F<256>::run(args...);
-    case 512: return F<512>::run(args...);
-    case 1024: return F<1024>::run(args...);
-    case 2048: return F<2048>::run(args...);
+    case 8: return F<8, 8>::run(args...);
+    case 16: return e16 ? F<16, 16>::run(args...) : F<16, 8>::run(args...);
+    case 32: return e16 ? F<32, 16>::run(args...) : F<32, 8>::run(args...);
+    case 64: return e16 ? F<64, 16>::run(args...) : F<64, 8>::run(args...);
+    case 128: return e16 ? F<128, 16>::run(args...) : F<128, 8>::run(args...);
+    case 256: return e16 ? F<256, 16>::run(args...) : F<256, 8>::run(args...);
+    case 512: return e16 ? F<512, 16>::run(args...) : F<512, 8>::run(args...);
+    case 1024: return e16 ? F<1024, 16>::run(args...) : F<1024, 8>::run(args...);
+    case 2048: return e16 ? F<2048, 16>::run(args...) : F<2048, 8>::run(args...);
     default:
       set_error("transform length %lld is not a power of two in [%d, %d]", (long long)n, PTX_MIN_N, PTX_MAX_N);
       return PTX_ERR_UNSUPPORTED;

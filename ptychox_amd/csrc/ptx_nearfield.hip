@@ -76,12 +76,12 @@ struct NfGammaRows {
 };
 
 // step: row iFFT / N -> A, loss partial per row, Gamma, row FFT (in place)
-template <int N>
+template <int N, int E>
 __global__ void __launch_bounds__(256)
     k_nf_step_rows(float2* __restrict__ tb, const float* __restrict__ measured, float* __restrict__ amp_out,
                    float* __restrict__ loss_part, int64_t nrows, float gscale, float inv_n) {
-  using Cf = RowCfg<N>;
-  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  using Cf = RowCfg<N, E>;
+  constexpr int T = Cf::T, S = Cf::S;
   __shared__ float2 lds[S * lds_len<N>()];
   __shared__ float red[256];
   const int seq = threadIdx.x / T;
@@ -118,33 +118,33 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-template <int N>
+template <int N, int E>
 struct TransferColsRun {
   static ptx_status run(float2* tb, const float2* transfer, int64_t frames, int64_t width, bool conj, hipStream_t s) {
-    launch_transfer_cols<N>(tb, transfer, frames, width, conj, s);
+    launch_transfer_cols<N, E>(tb, transfer, frames, width, conj, s);
     return check_launch(conj ? "nf transfer* cols" : "nf transfer cols", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct FieldRowsRun {
   static ptx_status run(const NfFieldRows& op, int64_t nrows, hipStream_t s) {
-    launch_rows<N, +1>(op, nrows, s);
+    launch_rows<N, E, +1>(op, nrows, s);
     return check_launch("nf field rows", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct GammaRowsRun {
   static ptx_status run(const NfGammaRows& op, int64_t nrows, hipStream_t s) {
-    launch_rows<N, -1>(op, nrows, s);
+    launch_rows<N, E, -1>(op, nrows, s);
     return check_launch("nf gamma rows", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct StepRowsRun {
   static ptx_status run(float2* tb, const float* measured, float* amp, float* loss_part, int64_t nrows, float gscale,
                         float inv_n, hipStream_t s) {
-    const int64_t blocks = (nrows + RowCfg<N>::S - 1) / RowCfg<N>::S;
-    k_nf_step_rows<N><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(tb, measured, amp, loss_part, nrows, gscale, inv_n);
+    const int64_t blocks = (nrows + RowCfg<N, E>::S - 1) / RowCfg<N, E>::S;
+    k_nf_step_rows<N, E><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(tb, measured, amp, loss_part, nrows, gscale, inv_n);
     return check_launch("nf step rows", s);
   }
 };
@@ -178,17 +178,17 @@ ptx_status ptx_nearfield_forward(const void* sample, int64_t H, int64_t W, const
   hipStream_t s = (hipStream_t)stream;
   timing_begin(s);
   ScanGatherRows<false> r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, pos, (float2*)field};
-  if ((st = dispatch_n<GatherRowsRun>(w, r1, B * h, s)) != PTX_OK) return st;
-  if ((st = dispatch_n<TransferColsRun>(h, (float2*)field, (const float2*)transfer, B, w, false, s)) != PTX_OK)
+  if ((st = dispatch_ne<GatherRowsRun>(PASS_ROW, w, r1, B * h, s)) != PTX_OK) return st;
+  if ((st = dispatch_ne<TransferColsRun>(PASS_COL, h, (float2*)field, (const float2*)transfer, B, w, false, s)) != PTX_OK)
     return st;
   NfFieldRows r3{(const float2*)field, (float2*)field, amplitude, 1.0f / (float)(h * w)};
-  return dispatch_n<FieldRowsRun>(w, r3, B * h, s);
+  return dispatch_ne<FieldRowsRun>(PASS_ROW, w, r3, B * h, s);
 }
 
 size_t ptx_nearfield_backward_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
   if (check_shapes("ptx_nearfield_backward_workspace", H, W, h, w, B) != PTX_OK) return 0;
   ScatterPlan sp{};
-  dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
+  dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
   return carve_scan(nullptr, H, W, h, w, B, flags, false, sp, 0).bytes;
 }
 
@@ -216,15 +216,15 @@ ptx_status ptx_nearfield_backward(const float* grad_output, const void* field, c
     return PTX_ERR_INVALID_ARGUMENT;
   }
   ScatterPlan sp{};
-  dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
+  dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
   const ScanWorkspace ws = carve_scan(workspace, H, W, h, w, B, flags, false, sp, 0);
   if (!workspace || ws_bytes < ws.bytes) {
     set_error("ptx_nearfield_backward: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
     return PTX_ERR_WORKSPACE;
   }
   NfGammaRows r1{(const float2*)field, grad_output, ws.tb};
-  if ((st = dispatch_n<GammaRowsRun>(w, r1, B * h, s)) != PTX_OK) return st;
-  if ((st = dispatch_n<TransferColsRun>(h, ws.tb, (const float2*)transfer, B, w, true, s)) != PTX_OK) return st;
+  if ((st = dispatch_ne<GammaRowsRun>(PASS_ROW, w, r1, B * h, s)) != PTX_OK) return st;
+  if ((st = dispatch_ne<TransferColsRun>(PASS_COL, h, ws.tb, (const float2*)transfer, B, w, true, s)) != PTX_OK) return st;
   return scan_scatter<false>(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, pos, B,
                              (float2*)grad_sample, (float2*)grad_probe, flags, 1.0f / (float)(h * w), s);
 }
@@ -232,7 +232,7 @@ ptx_status ptx_nearfield_backward(const float* grad_output, const void* field, c
 size_t ptx_nearfield_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
   if (check_shapes("ptx_nearfield_step_workspace", H, W, h, w, B) != PTX_OK) return 0;
   ScatterPlan sp{};
-  dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
+  dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
   return carve_scan(nullptr, H, W, h, w, B, flags, true, sp, h).bytes;
 }
 
@@ -260,7 +260,7 @@ ptx_status ptx_nearfield_step(const void* sample, int64_t H, int64_t W, const vo
     return PTX_ERR_INVALID_ARGUMENT;
   }
   ScatterPlan sp{};
-  dispatch_n<ScatterPlanQuery>(w, h, B, &sp);
+  dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
   const ScanWorkspace ws = carve_scan(workspace, H, W, h, w, B, flags, true, sp, h);
   if (!workspace || ws_bytes < ws.bytes) {
     set_error("ptx_nearfield_step: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
@@ -268,14 +268,14 @@ ptx_status ptx_nearfield_step(const void* sample, int64_t H, int64_t W, const vo
   }
   const float inv_n = 1.0f / (float)(h * w);
   ScanGatherRows<false> r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, pos, ws.tb};
-  if ((st = dispatch_n<GatherRowsRun>(w, r1, B * h, s)) != PTX_OK) return st;
-  if ((st = dispatch_n<TransferColsRun>(h, ws.tb, (const float2*)transfer, B, w, false, s)) != PTX_OK) return st;
-  if ((st = dispatch_n<StepRowsRun>(w, ws.tb, measured, amplitude, ws.loss_part, B * h, 2.0f * loss_scale, inv_n,
+  if ((st = dispatch_ne<GatherRowsRun>(PASS_ROW, w, r1, B * h, s)) != PTX_OK) return st;
+  if ((st = dispatch_ne<TransferColsRun>(PASS_COL, h, ws.tb, (const float2*)transfer, B, w, false, s)) != PTX_OK) return st;
+  if ((st = dispatch_ne<StepRowsRun>(PASS_ROW, w, ws.tb, measured, amplitude, ws.loss_part, B * h, 2.0f * loss_scale, inv_n,
                                     s)) != PTX_OK)
     return st;
   loss_rows(ws.loss_part, h, B, inv_n, loss_per_position, s);
   if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
-  if ((st = dispatch_n<TransferColsRun>(h, ws.tb, (const float2*)transfer, B, w, true, s)) != PTX_OK) return st;
+  if ((st = dispatch_ne<TransferColsRun>(PASS_COL, h, ws.tb, (const float2*)transfer, B, w, true, s)) != PTX_OK) return st;
   return scan_scatter<false>(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, pos, B,
                              (float2*)grad_sample, (float2*)grad_probe, flags, inv_n, s);
 }

@@ -49,14 +49,14 @@ struct ScanGatherRows {
 };
 
 // ---------------------------------------------------------------- row iFFT, (sign | scale), object scatter, probe partials
-template <int N, bool SIGN>
+template <int N, int E, bool SIGN>
 __global__ void __launch_bounds__(256)
     k_scan_scatter_rows(const float2* __restrict__ tb, const float2* __restrict__ sample, int64_t H, int64_t W,
                         const float2* __restrict__ probe, int h, const int64_t* __restrict__ pos, int64_t B,
                         int chunk, int ngroups, float2* __restrict__ gobj, float2* __restrict__ contrib,
                         float2* __restrict__ gp_part, float scale) {
-  using Cf = RowCfg<N>;
-  constexpr int E = Cf::E, T = Cf::T, S = Cf::S;
+  using Cf = RowCfg<N, E>;
+  constexpr int T = Cf::T, S = Cf::S;
   __shared__ float2 lds[S * lds_len<N>()];
   const int seq = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -106,18 +106,18 @@ __global__ void __launch_bounds__(256)
 
 // ---------------------------------------------------------------- host side
 
-template <int N>
+template <int N, int E>
 struct GatherRowsRun {
   template <bool SIGN>
   static ptx_status run(const ScanGatherRows<SIGN>& op, int64_t nrows, hipStream_t s) {
-    launch_rows<N, -1>(op, nrows, s);
+    launch_rows<N, E, -1>(op, nrows, s);
     return check_launch(SIGN ? "base gather rows" : "scan gather rows", s);
   }
 };
-template <int N>
+template <int N, int E>
 struct ColTilesQuery {
   static ptx_status run(int64_t width, int64_t* tiles) {
-    using Cf = ColCfg<N>;
+    using Cf = ColCfg<N, E>;
     *tiles = width >= Cf::CMAX ? width / Cf::CMAX : width / 8;
     return PTX_OK;
   }
@@ -130,10 +130,10 @@ struct ScatterPlan {
   int nchunks;  // probe partial slices
 };
 
-template <int N>
+template <int N, int E>
 struct ScatterPlanQuery {
   static ptx_status run(int64_t h, int64_t B, ScatterPlan* p) {
-    p->S = RowCfg<N>::S;
+    p->S = RowCfg<N, E>::S;
     p->ngroups = (int)((h + p->S - 1) / p->S);
     // aim for ~1024 work-groups; each probe-partial slice covers `chunk` views
     int64_t want = (1024 + p->ngroups - 1) / p->ngroups;
@@ -147,13 +147,13 @@ struct ScatterPlanQuery {
   }
 };
 
-template <int N>
+template <int N, int E>
 struct ScatterRowsRun {
   template <bool SIGN>
   static ptx_status run(const float2* tb, const float2* sample, int64_t H, int64_t W, const float2* probe, int h,
                         const int64_t* pos, int64_t B, const ScatterPlan& p, float2* gobj, float2* contrib,
                         float2* gp_part, float scale, hipStream_t s, std::integral_constant<bool, SIGN>) {
-    k_scan_scatter_rows<N, SIGN><<<dim3((unsigned)(p.ngroups * p.nchunks)), dim3(256), 0, s>>>(
+    k_scan_scatter_rows<N, E, SIGN><<<dim3((unsigned)(p.ngroups * p.nchunks)), dim3(256), 0, s>>>(
         tb, sample, H, W, probe, h, pos, B, p.chunk, p.ngroups, gobj, contrib, gp_part, scale);
     return check_launch(SIGN ? "base scatter rows" : "scan scatter rows", s);
   }
@@ -214,7 +214,7 @@ ptx_status scan_scatter(const ScanWorkspace& ws, const ScatterPlan& sp, const fl
     fill_zero2(gs, H * W, s);
     if ((st = check_launch("zero fill", s)) != PTX_OK) return st;
   }
-  st = dispatch_n<ScatterRowsRun>(w, ws.tb, sample, H, W, probe, (int)h, pos, B, sp, gs, det ? ws.tb : nullptr,
+  st = dispatch_ne<ScatterRowsRun>(PASS_ROW_LOOP, w, ws.tb, sample, H, W, probe, (int)h, pos, B, sp, gs, det ? ws.tb : nullptr,
                                   ws.gp_part, scale, s, std::integral_constant<bool, SIGN>{});
   if (st != PTX_OK) return st;
   sum_parts2(ws.gp_part, sp.nchunks, h * w, gp, acc ? 1 : 0, s);

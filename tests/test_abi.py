@@ -89,3 +89,25 @@ def test_timing_hook_arms_and_disarms(lib):
     assert lib.lib().ptx_timing_count() == 0
     with pytest.raises(ValueError):
         lib.call("ptx_timing_events", None, 4)
+
+
+def test_tuning_table_roundtrip(lib):
+    h = lib.lib()
+    assert h.ptx_tune_get(0, 512) in (8, 16)
+    lib.call("ptx_tune_set", 2, 512, 16)
+    assert h.ptx_tune_get(2, 512) == 16
+    lib.call("ptx_tune_set", 2, 512, 8)
+    assert h.ptx_tune_get(2, 512) == 8
+    lib.call("ptx_tune_set", 2, 512, 0)  # back to the default
+    with pytest.raises(ValueError):
+        lib.call("ptx_tune_set", 7, 512, 8)
+    with pytest.raises(ValueError):
+        lib.call("ptx_tune_set", 0, 512, 4)
+    with pytest.raises(ValueError):
+        lib.call("ptx_tune_set", 0, 8, 16)
+    from ptychox_amd import autotune
+
+    t = autotune.table(autotune.keys_for_scan((512, 512)))
+    assert set(t) == {"row:512", "row_loop:512", "col:512"}
+    autotune.apply(t)
+    assert autotune.table(autotune.keys_for_scan((512, 512))) == t
