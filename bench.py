@@ -46,6 +46,8 @@ def parse_args(argv=None):
                     help="object gradient by float atomics instead of the (default) sorted-segment gather")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in FFT schedule (skip the autotuner)")
+    ap.add_argument("--no-modalities", action="store_true",
+                    help="skip the Fourier / near-field / coded full-size measurements")
     return ap.parse_args(argv)
 
 
@@ -67,6 +69,112 @@ def kernel_bytes(label: str, B: int, N: int, H: int, W: int, nparts: int, atomic
         "zero fill": 8 * H * W,
     }
     return table.get(label)
+
+
+# Algorithmic HBM bytes per pattern of the fused forward + loss + adjoint:
+# per-pattern boundary operands moved once (patch / sub-spectrum read for the
+# forward and again for the adjoint's product rule, amplitude write, gradient
+# read, object / spectrum accumulator read-modify-write; shared operands such
+# as the full-frame object FFT amortised over the batch) plus one 16-byte
+# round trip per pixel for every 2-D FFT whose frame exceeds the LDS.
+B_ALG = {
+    "fourier 1024^2 spectrum, 256^2 pupil, 225 LEDs": 5.024e6,
+    "near-field 1024^2 object, 512^2 probe, 64 positions": 27.394e6,
+    "coded 1024^2, 49 translations, ds=1": 145.345e6,
+}
+
+
+def _timed(fn, reps):
+    import torch
+
+    fn()
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def measure_modalities(dev, args) -> dict:
+    """Fused-step throughput of the other three operators at their full sizes,
+    beside the unfused torch path on the same GPU."""
+    import torch
+
+    from ptychox_amd import autotune, ptychoX_coded, ptychoX_fourier, ptychoX_nearfield
+    from ptychox_amd.coded import CodedStep
+    from ptychox_amd.fpm import FpmStep
+    from ptychox_amd.nearfield import NearFieldStep
+    from ptychox_amd.synthetic import coded_dataset, fpm_dataset, nearfield_dataset
+
+    out = {}
+    mse = torch.nn.functional.mse_loss
+
+    def record(name, B, fused_fn, plain_fn, keys):
+        if not args.no_tune:
+            autotune.autotune(fused_fn, keys, reps=5)
+        ms = _timed(fused_fn, 20)
+        pms = _timed(plain_fn, 3)
+        v = B / (ms / 1e3)
+        frac = B_ALG[name] * v / (HBM_PEAK_GBS * 1e9)
+        out[name] = {"value": round(v, 1), "unit": "patterns/s", "ms_per_step": round(ms, 4),
+                     "plain_value": round(B / (pms / 1e3), 1), "speedup_of_fused": round(pms / ms, 2),
+                     "bytes_per_pattern": B_ALG[name], "path_roofline_frac": round(frac, 4)}
+        print(f"[bench] {name}: {ms:.3f} ms/step, {v:.0f} patterns/s ({frac:.1%} of HBM roofline), "
+              f"plain {pms:.3f} ms", file=sys.stderr, flush=True)
+
+    # Fourier ptychography
+    d = fpm_dataset(1024, 256, 48, 15, 40, dev, seed=0)
+    B = d.leds.shape[0]
+    st = FpmStep(d.object_field.shape, d.pupil.shape, B, dev)
+    o, q = d.object_field.clone(), d.pupil.clone()
+    go, gq = torch.empty_like(o), torch.empty_like(q)
+    op, qp = o.clone().requires_grad_(True), q.clone().requires_grad_(True)
+
+    def fpm_plain():
+        op.grad = qp.grad = None
+        mse(ptychoX_fourier(op, qp, d.leds, use_cuda_kernels=False), d.measured).backward()
+
+    record("fourier 1024^2 spectrum, 256^2 pupil, 225 LEDs", B, lambda: st(o, q, d.leds, d.measured, go, gq),
+           fpm_plain, autotune.keys_for_fpm(o.shape, q.shape))
+    del d, st
+    # near-field
+    d = nearfield_dataset(1024, 512, 64, 32, 20.0, 1e4, dev, seed=0)
+    B = d.positions.shape[0]
+    st = NearFieldStep(d.sample.shape, d.probe.shape, B, dev)
+    s_, p_ = d.sample.clone(), d.probe.clone()
+    gs, gp = torch.empty_like(s_), torch.empty_like(p_)
+    sp, pp = s_.clone().requires_grad_(True), p_.clone().requires_grad_(True)
+
+    def nf_plain():
+        sp.grad = pp.grad = None
+        mse(ptychoX_nearfield(sp, pp, d.transfer, d.positions, use_cuda_kernels=False), d.measured).backward()
+
+    record("near-field 1024^2 object, 512^2 probe, 64 positions", B,
+           lambda: st(s_, p_, d.transfer, d.positions, d.measured, gs, gp), nf_plain,
+           autotune.keys_for_scan(p_.shape))
+    del d, st
+    # coded
+    d = coded_dataset(1024, 7, 8, 256.0**2 / 20.0, 1e4, dev, seed=0)
+    B = d.shifts.shape[0]
+    st = CodedStep(d.object_field.shape, B, 1, dev)
+    o, c = d.object_field.clone(), d.coded_surface.clone()
+    go, gc = torch.empty_like(o), torch.empty_like(c)
+    op, cp = o.clone().requires_grad_(True), c.clone().requires_grad_(True)
+
+    def coded_plain():
+        op.grad = cp.grad = None
+        mse(ptychoX_coded(op, cp, d.transfer_1, d.transfer_2, d.shifts, 1, use_cuda_kernels=False),
+            d.measured).backward()
+
+    record("coded 1024^2, 49 translations, ds=1", B,
+           lambda: st(o, c, d.transfer_1, d.transfer_2, d.shifts, d.measured, go, gc), coded_plain,
+           autotune.keys_for_coded(o.shape))
+    return out
 
 
 def main(argv=None) -> int:
@@ -224,6 +332,8 @@ def main(argv=None) -> int:
                          f"per-view Python loops as in the reference, {t_cpu:.1f} s"}
         print(f"[bench] cpu baseline: {cpu['value']} patterns/s on {threads} threads", file=sys.stderr, flush=True)
 
+    modalities = None if args.no_modalities else measure_modalities(dev, args)
+
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -250,6 +360,7 @@ def main(argv=None) -> int:
         "tuning": tuning,
         "plain_path": plain,
         "cpu_baseline": cpu,
+        "other_workloads": modalities,
     }
     print(json.dumps(out), flush=True)
     return 0

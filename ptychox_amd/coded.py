@@ -172,14 +172,18 @@ class CodedStep:
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
-        nbytes = _lib.query("ptx_coded_workspace", self.H, self.W, self.batch, self.ds, 1)
-        self.ws = _lib.workspace(nbytes, self.device)
+        self.ws = _lib.workspace(self._ws_bytes(), self.device)
         self.loss = torch.empty(self.batch, dtype=torch.float32, device=self.device)
         self.amplitude = (
             torch.empty((self.batch, self.H // self.ds, self.W // self.ds), dtype=torch.float32, device=self.device)
             if keep_amplitude
             else None
         )
+
+    def _ws_bytes(self) -> int:
+        # the size depends on the FFT schedule in the tuning table, so it is
+        # re-queried (host-side, no GPU work) before every call
+        return _lib.query("ptx_coded_workspace", self.H, self.W, self.batch, self.ds, 1)
 
     def __call__(self, object_field, coded_surface, transfer_1, transfer_2, shifts_xy, measured, grad_object,
                  grad_coded, loss_scale=None, batch=None, accumulate=False):
@@ -197,6 +201,8 @@ class CodedStep:
         if loss_scale is None:
             loss_scale = 1.0 / (B * (self.H // self.ds) * (self.W // self.ds))
         flags = _lib.PTX_FLAG_ACCUMULATE if accumulate else 0
+        if self._ws_bytes() > self.ws.numel():
+            self.ws = _lib.workspace(self._ws_bytes(), self.device)
         _lib.call(
             "ptx_coded_step",
             _lib.ptr(object_field), self.H, self.W, _lib.ptr(coded_surface), _lib.ptr(transfer_1),

@@ -117,6 +117,11 @@ static __global__ void __launch_bounds__(256) k_tile_lists(const int64_t* __rest
 
 // grad[Y, X] (+)= sum over listed views b (ascending) of contrib[b, cy, cx] with
 // (cy, cx) = ((Y - oy_b) mod H, (X - ox_b) mod W) inside the h x w window.
+// The work-group first stages (b, oy_b, ox_b) of up to kViewsPerPass listed
+// views in LDS, then walks them two at a time so that sixteen independent
+// 8-byte loads per thread are in flight (the per-view latency was the limit).
+constexpr int kViewsPerPass = 256;
+
 static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __restrict__ contrib, int h, int w,
                                                             const int64_t* __restrict__ pos, int64_t batch,
                                                             int64_t off_y, int64_t off_x,
@@ -126,35 +131,62 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
                                                             int accumulate) {
   constexpr int RPT = kTileH * kTileW / 256;  // rows per thread
   constexpr int RSTEP = 256 / kTileW;
+  __shared__ int s_b[kViewsPerPass];
+  __shared__ int s_oy[kViewsPerPass];
+  __shared__ int s_ox[kViewsPerPass];
   const int64_t tile = blockIdx.x;
   const int64_t ty0 = (tile / tiles_x) * kTileH, tx0 = (tile % tiles_x) * kTileW;
   const int64_t X = tx0 + (threadIdx.x % kTileW);
   const int64_t Y0 = ty0 + threadIdx.x / kTileW;
+  const bool xin = X < W;
   float2 acc[RPT];
 #pragma unroll
   for (int j = 0; j < RPT; ++j) acc[j] = c_zero();
   const int n = counts[tile];
   const int* list = lists + tile * batch;
-  for (int k = 0; k < n; ++k) {
-    const int64_t b = list[k];
-    // window origin in [0, H) x [0, W); offsets from it fold back by one
-    // conditional add (no integer division on the per-pixel path)
-    const int64_t oy = pmod(pos[2 * b + 1] + off_y, H), ox = pmod(pos[2 * b] + off_x, W);
-    int64_t cx = X - ox;
-    if (cx < 0) cx += W;
-    if (X >= W || cx >= w) continue;
-    const float2* src = contrib + (b * h) * w + cx;
-    int64_t cy = Y0 - oy;
-    if (cy < 0) cy += H;
+  for (int base = 0; base < n; base += kViewsPerPass) {
+    const int m = n - base < kViewsPerPass ? n - base : kViewsPerPass;
+    __syncthreads();  // previous pass finished reading the staged views
+    for (int k = threadIdx.x; k < m; k += 256) {
+      const int64_t b = list[base + k];
+      s_b[k] = (int)b;
+      s_oy[k] = (int)pmod(pos[2 * b + 1] + off_y, H);
+      s_ox[k] = (int)pmod(pos[2 * b] + off_x, W);
+    }
+    __syncthreads();
+    for (int k = 0; k < m; k += 2) {
+      float2 t0[RPT], t1[RPT];
+      bool u0[RPT], u1[RPT];
 #pragma unroll
-    for (int j = 0; j < RPT; ++j) {
-      const int64_t Y = Y0 + j * RSTEP;
-      if (Y < H && cy < h) acc[j] = c_add(acc[j], src[cy * w]);
-      cy += RSTEP;
-      if (cy >= H) cy -= H;
+      for (int v = 0; v < 2; ++v) {
+        float2* tv = v == 0 ? t0 : t1;
+        bool* uv = v == 0 ? u0 : u1;
+        const int kk = k + v;
+        const bool have = kk < m;
+        const int64_t b = have ? s_b[kk] : 0;
+        int64_t cx = X - (have ? s_ox[kk] : 0);
+        if (cx < 0) cx += W;
+        int64_t cy = Y0 - (have ? s_oy[kk] : 0);
+        if (cy < 0) cy += H;
+        const bool colok = have && xin && cx < w;
+        const float2* src = contrib + (b * h) * w + cx;
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+          const int64_t Y = Y0 + j * RSTEP;
+          uv[j] = colok && Y < H && cy < h;
+          tv[j] = uv[j] ? src[cy * w] : c_zero();
+          cy += RSTEP;
+          if (cy >= H) cy -= H;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) {
+        if (u0[j]) acc[j] = c_add(acc[j], t0[j]);  // ascending view order
+        if (u1[j]) acc[j] = c_add(acc[j], t1[j]);
+      }
     }
   }
-  if (X >= W) return;
+  if (!xin) return;
 #pragma unroll
   for (int j = 0; j < RPT; ++j) {
     const int64_t Y = Y0 + j * RSTEP;

@@ -39,6 +39,10 @@ __device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float
 __device__ __forceinline__ float2 c_neg(float2 a) { return make_float2(-a.x, -a.y); }
 __device__ __forceinline__ float c_abs(float2 a) { return sqrtf(a.x * a.x + a.y * a.y); }
 __device__ __forceinline__ float2 c_zero() { return make_float2(0.0f, 0.0f); }
+// 1/x by the hardware reciprocal (v_rcp_f32, ~1 ulp) for the normalising
+// divides of the loss / phase epilogues: the IEEE divide sequence costs ~10
+// instructions and several registers per element.  x >= FLT_EPSILON there.
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // ---------------------------------------------------------------- compile-time twiddle table
 

@@ -108,20 +108,20 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   const int t = threadIdx.x / C;
   const int64_t frame = blockIdx.x / tiles_per_frame;
   const int64_t col = (blockIdx.x % tiles_per_frame) * C + c;
-  float2* p = tb + frame * N * width + col;
+  float2* p = tb + frame * N * width;
   float2 v[E];
 #pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = p[(int64_t)(t + q * T) * width];
+  for (int q = 0; q < E; ++q) v[q] = p[(t + q * T) * (int)width + (int)col];
   fft_seq<N, E, -1>(v, t, &lds[c * lds_len<N>()]);
 #pragma unroll
   for (int q = 0; q < E; ++q) {
-    const float2 tr = transfer[(int64_t)(t + q * T) * width + col];
+    const float2 tr = transfer[(t + q * T) * (int)width + (int)col];
     v[q] = CONJ ? c_mulc(v[q], tr) : c_mul(v[q], tr);
   }
   __syncthreads();  // second transform reuses the LDS region
   fft_seq<N, E, +1>(v, t, &lds[c * lds_len<N>()]);
 #pragma unroll
-  for (int q = 0; q < E; ++q) p[(int64_t)(t + q * T) * width] = v[q];
+  for (int q = 0; q < E; ++q) p[(t + q * T) * (int)width + (int)col] = v[q];
 }
 
 // host launcher (check_launch is the caller's)
@@ -198,16 +198,16 @@ struct ColsInPlace {
   template <int N, int E>
   __device__ __forceinline__ void load(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    const float2* p = data + frame * N * width + col;
+    const float2* p = data + frame * N * width;
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = p[(int64_t)(t + q * T) * width];
+    for (int q = 0; q < E; ++q) v[q] = p[(t + q * T) * (int)width + (int)col];
   }
   template <int N, int E>
   __device__ __forceinline__ void store(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    float2* p = data + frame * N * width + col;
+    float2* p = data + frame * N * width;
 #pragma unroll
-    for (int q = 0; q < E; ++q) p[(int64_t)(t + q * T) * width] = c_scale(v[q], scale);
+    for (int q = 0; q < E; ++q) p[(t + q * T) * (int)width + (int)col] = c_scale(v[q], scale);
   }
 };
 

@@ -28,17 +28,17 @@ struct BaseFarFieldCols {
   template <int N, int E>
   __device__ __forceinline__ void load(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    const float2* p = ff + frame * N * width + col;
+    const float2* p = ff + frame * N * width;
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = p[(int64_t)(t + q * T) * width];
+    for (int q = 0; q < E; ++q) v[q] = p[(t + q * T) * (int)width + (int)col];
   }
   template <int N, int E>
   __device__ __forceinline__ void store(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    const int64_t off = frame * N * width + col;
+    const int64_t off = frame * N * width;  // frame base (uniform)
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      const int64_t k = off + (int64_t)(t + q * T) * width;
+      const int64_t k = off + ((t + q * T) * (int)width + (int)col);
       ff[k] = v[q];
       amp[k] = c_abs(v[q]);
     }
@@ -54,21 +54,21 @@ struct BaseGammaCols {
   template <int N, int E>
   __device__ __forceinline__ void load(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    const int64_t off = frame * N * width + col;
+    const int64_t off = frame * N * width;  // frame base (uniform)
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      const int64_t k = off + (int64_t)(t + q * T) * width;
+      const int64_t k = off + ((t + q * T) * (int)width + (int)col);
       const float2 phi = ff[k];
-      const float s = g[k] / fmaxf(c_abs(phi), kFltEps);
+      const float s = g[k] * fast_rcp(fmaxf(c_abs(phi), kFltEps));
       v[q] = c_scale(phi, s);
     }
   }
   template <int N, int E>
   __device__ __forceinline__ void store(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    float2* p = out + frame * N * width + col;
+    float2* p = out + frame * N * width;
 #pragma unroll
-    for (int q = 0; q < E; ++q) p[(int64_t)(t + q * T) * width] = v[q];
+    for (int q = 0; q < E; ++q) p[(t + q * T) * (int)width + (int)col] = v[q];
   }
 };
 
@@ -86,26 +86,26 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   const int64_t frame = blockIdx.x / tiles_per_frame;
   const int64_t tile = blockIdx.x % tiles_per_frame;
   const int64_t col = tile * C + c;
-  const int64_t off = frame * N * width + col;
+  const int64_t off = frame * N * width;  // frame base (uniform)
   float2 v[E];
 #pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = tb[off + (int64_t)(t + q * T) * width];
+  for (int q = 0; q < E; ++q) v[q] = tb[off + ((t + q * T) * (int)width + (int)col)];
   fft_seq<N, E, -1>(v, t, &lds[c * lds_len<N>()]);
   float lsum = 0.0f;
 #pragma unroll
   for (int q = 0; q < E; ++q) {
-    const int64_t k = off + (int64_t)(t + q * T) * width;
+    const int64_t k = off + ((t + q * T) * (int)width + (int)col);
     const float a = c_abs(v[q]);
     const float d = a - measured[k];
     lsum += d * d;
     if (amp_out) amp_out[k] = a;
-    v[q] = c_scale(v[q], gscale * d / fmaxf(a, kFltEps));
+    v[q] = c_scale(v[q], gscale * d * fast_rcp(fmaxf(a, kFltEps)));
   }
   const float tot = block_sum(lsum, red);  // contains the barrier that also orders the LDS reuse below
   if (threadIdx.x == 0) loss_part[frame * tiles_per_frame + tile] = tot;
   fft_seq<N, E, +1>(v, t, &lds[c * lds_len<N>()]);
 #pragma unroll
-  for (int q = 0; q < E; ++q) tb[off + (int64_t)(t + q * T) * width] = v[q];
+  for (int q = 0; q < E; ++q) tb[off + ((t + q * T) * (int)width + (int)col)] = v[q];
 }
 
 // ---------------------------------------------------------------- host launchers

@@ -76,9 +76,9 @@ struct CodedSpecCols {
   template <int N, int E>
   __device__ __forceinline__ void store(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    float2* p = out + frame * N * width + col;
+    float2* p = out + frame * N * width;
 #pragma unroll
-    for (int q = 0; q < E; ++q) p[(int64_t)(t + q * T) * width] = v[q];
+    for (int q = 0; q < E; ++q) p[(t + q * T) * (int)width + (int)col] = v[q];
   }
 };
 
@@ -162,10 +162,10 @@ static __global__ void __launch_bounds__(256)
   const float a = sqrtf(fmaxf(s * inv_cell, 0.0f));
   if (amp) amp[i] = a;
   if (mode == 1) {
-    sc[i] = g[i] * inv_cell / fmaxf(a, kCodedAmpEps);
+    sc[i] = g[i] * inv_cell * fast_rcp(fmaxf(a, kCodedAmpEps));
   } else if (mode == 2) {
     const float d = a - g[i];
-    sc[i] = gscale * d * inv_cell / fmaxf(a, kCodedAmpEps);
+    sc[i] = gscale * d * inv_cell * fast_rcp(fmaxf(a, kCodedAmpEps));
     loss_part[i] = d * d;
   }
 }
@@ -191,7 +191,7 @@ struct CodedGradZRows {
         v[q] = c_zero();
       } else if (ds == 1) {
         const float2 e = z[row * N + i];
-        v[q] = c_scale(e, srow[i] / fmaxf(c_abs(e), kCodedAmpEps));
+        v[q] = c_scale(e, srow[i] * fast_rcp(fmaxf(c_abs(e), kCodedAmpEps)));
       } else {
         v[q] = c_scale(z[row * N + i], srow[i / ds]);
       }
@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(256)
     const float d = valid ? a - measured[k] : 0.0f;
     lsum += d * d;
     if (valid && amp_out) amp_out[k] = a;
-    v[q] = c_scale(e, gscale * d / fmaxf(a, kCodedAmpEps));
+    v[q] = c_scale(e, gscale * d * fast_rcp(fmaxf(a, kCodedAmpEps)));
   }
   red[threadIdx.x] = lsum;
   __syncthreads();
@@ -317,14 +317,14 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   float2 hc[E], acc[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) {
-    hc[q] = h1[(int64_t)(t + q * T) * width + col];
+    hc[q] = h1[(t + q * T) * (int)width + (int)col];
     acc[q] = c_zero();
   }
   for (int64_t b = b0; b < b1; ++b) {
-    const float2* p = tb + b * N * width + col;
+    const float2* p = tb + b * N * width;  // frame base; the lane adds its column
     float2 v[E];
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = p[(int64_t)(t + q * T) * width];
+    for (int q = 0; q < E; ++q) v[q] = p[(t + q * T) * (int)width + (int)col];
     fft_seq<N, E, -1>(v, t, &lds[c * lds_len<N>()]);
     const float2 rx = ex[b * width + col];
 #pragma unroll
@@ -335,9 +335,9 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     }
     __syncthreads();
   }
-  float2* d = gs_part + (int64_t)ch * N * width + col;
+  float2* d = gs_part + (int64_t)ch * N * width;  // slice base; the lane adds its column
 #pragma unroll
-  for (int q = 0; q < E; ++q) d[(int64_t)(t + q * T) * width] = acc[q];
+  for (int q = 0; q < E; ++q) d[(t + q * T) * (int)width + (int)col] = acc[q];
 }
 
 // ---------------------------------------------------------------- host launchers
@@ -464,14 +464,14 @@ struct ColsInto {
   __device__ __forceinline__ void load(int64_t, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = in[(int64_t)(t + q * T) * width + col];
+    for (int q = 0; q < E; ++q) v[q] = in[(t + q * T) * (int)width + (int)col];
   }
   template <int N, int E>
   __device__ __forceinline__ void store(int64_t, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      float2* d = out + (int64_t)(t + q * T) * width + col;
+      float2* d = out + (t + q * T) * (int)width + (int)col;
       *d = accumulate ? c_add(*d, v[q]) : v[q];
     }
   }

@@ -67,20 +67,20 @@ struct FpmImageCols {
   template <int N, int E>
   __device__ __forceinline__ void load(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    const float2* p = field + frame * N * width + col;
+    const float2* p = field + frame * N * width;
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = p[(int64_t)(t + q * T) * width];
+    for (int q = 0; q < E; ++q) v[q] = p[(t + q * T) * (int)width + (int)col];
   }
   template <int N, int E>
   __device__ __forceinline__ void store(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    const int64_t off = frame * N * width + col;
+    const int64_t off = frame * N * width;  // frame base (uniform)
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int r = t + q * T;
       const float s = ((r + col) & 1) ? -inv_n : inv_n;
       const float2 e = c_scale(v[q], s);
-      const int64_t k = off + (int64_t)r * width;
+      const int64_t k = off + (r * (int)width + (int)col);
       field[k] = e;
       amp[k] = c_abs(e);
     }
@@ -96,13 +96,13 @@ struct FpmGammaCols {
   template <int N, int E>
   __device__ __forceinline__ void load(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    const int64_t off = frame * N * width + col;
+    const int64_t off = frame * N * width;  // frame base (uniform)
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int r = t + q * T;
-      const int64_t k = off + (int64_t)r * width;
+      const int64_t k = off + (r * (int)width + (int)col);
       const float2 e = field[k];
-      float s = g[k] / fmaxf(c_abs(e), kFltEps);
+      float s = g[k] * fast_rcp(fmaxf(c_abs(e), kFltEps));
       if ((r + col) & 1) s = -s;
       v[q] = c_scale(e, s);
     }
@@ -110,9 +110,9 @@ struct FpmGammaCols {
   template <int N, int E>
   __device__ __forceinline__ void store(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    float2* p = out + frame * N * width + col;
+    float2* p = out + frame * N * width;
 #pragma unroll
-    for (int q = 0; q < E; ++q) p[(int64_t)(t + q * T) * width] = v[q];
+    for (int q = 0; q < E; ++q) p[(t + q * T) * (int)width + (int)col] = v[q];
   }
 };
 
@@ -131,26 +131,26 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   const int64_t frame = blockIdx.x / tiles_per_frame;
   const int64_t tile = blockIdx.x % tiles_per_frame;
   const int64_t col = tile * C + c;
-  const int64_t off = frame * N * width + col;
+  const int64_t off = frame * N * width;  // frame base (uniform)
   float2 v[E];
 #pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = tb[off + (int64_t)(t + q * T) * width];
+  for (int q = 0; q < E; ++q) v[q] = tb[off + ((t + q * T) * (int)width + (int)col)];
   fft_seq<N, E, +1>(v, t, &lds[c * lds_len<N>()]);
   float lsum = 0.0f;
 #pragma unroll
   for (int q = 0; q < E; ++q) {
-    const int64_t k = off + (int64_t)(t + q * T) * width;
+    const int64_t k = off + ((t + q * T) * (int)width + (int)col);
     const float a = c_abs(v[q]) * inv_n;
     const float d = a - measured[k];
     lsum += d * d;
     if (amp_out) amp_out[k] = a;
-    v[q] = c_scale(v[q], gscale * d * inv_n / fmaxf(a, kFltEps));
+    v[q] = c_scale(v[q], gscale * d * inv_n * fast_rcp(fmaxf(a, kFltEps)));
   }
   const float tot = block_sum(lsum, red);
   if (threadIdx.x == 0) loss_part[frame * tiles_per_frame + tile] = tot;
   fft_seq<N, E, -1>(v, t, &lds[c * lds_len<N>()]);
 #pragma unroll
-  for (int q = 0; q < E; ++q) tb[off + (int64_t)(t + q * T) * width] = v[q];
+  for (int q = 0; q < E; ++q) tb[off + ((t + q * T) * (int)width + (int)col)] = v[q];
 }
 
 // ---------------------------------------------------------------- R2: row FFT, pupil partials, spectrum scatter
@@ -219,17 +219,17 @@ struct ColsToOut {
   template <int N, int E>
   __device__ __forceinline__ void load(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    const float2* p = in + frame * N * width + col;
+    const float2* p = in + frame * N * width;
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = p[(int64_t)(t + q * T) * width];
+    for (int q = 0; q < E; ++q) v[q] = p[(t + q * T) * (int)width + (int)col];
   }
   template <int N, int E>
   __device__ __forceinline__ void store(int64_t frame, int64_t col, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    float2* p = out + frame * N * width + col;
+    float2* p = out + frame * N * width;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      float2* d = p + (int64_t)(t + q * T) * width;
+      float2* d = p + (t + q * T) * (int)width + (int)col;
       *d = accumulate ? c_add(*d, v[q]) : v[q];
     }
   }

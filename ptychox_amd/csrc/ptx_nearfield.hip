@@ -59,7 +59,7 @@ struct NfGammaRows {
       const int64_t k = row * N + t + q * T;
       if (valid) {
         const float2 e = field[k];
-        v[q] = c_scale(e, g[k] / fmaxf(c_abs(e), kFltEps));
+        v[q] = c_scale(e, g[k] * fast_rcp(fmaxf(c_abs(e), kFltEps)));
       } else {
         v[q] = c_zero();
       }
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256)
     const float d = valid ? a - measured[k] : 0.0f;
     lsum += d * d;
     if (valid && amp_out) amp_out[k] = a;
-    v[q] = c_scale(e, gscale * d / fmaxf(a, kFltEps));
+    v[q] = c_scale(e, gscale * d * fast_rcp(fmaxf(a, kFltEps)));
   }
   red[threadIdx.x] = lsum;
   __syncthreads();  // also orders the LDS reuse of the second transform

@@ -171,14 +171,18 @@ class NearFieldStep:
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
         self.flags = scatter_flags(scatter)
-        nbytes = _lib.query("ptx_nearfield_step_workspace", self.H, self.W, self.h, self.w, self.batch, self.flags)
-        self.ws = _lib.workspace(nbytes, self.device)
+        self.ws = _lib.workspace(self._ws_bytes(), self.device)
         self.loss = torch.empty(self.batch, dtype=torch.float32, device=self.device)
         self.amplitude = (
             torch.empty((self.batch, self.h, self.w), dtype=torch.float32, device=self.device)
             if keep_amplitude
             else None
         )
+
+    def _ws_bytes(self) -> int:
+        # the size depends on the FFT schedule in the tuning table, so it is
+        # re-queried (host-side, no GPU work) before every call
+        return _lib.query("ptx_nearfield_step_workspace", self.H, self.W, self.h, self.w, self.batch, self.flags)
 
     def __call__(self, sample, probe, transfer, positions_xy, measured, grad_sample, grad_probe, loss_scale=None,
                  batch=None, accumulate=False):
@@ -194,6 +198,8 @@ class NearFieldStep:
         if loss_scale is None:
             loss_scale = 1.0 / (B * self.h * self.w)
         flags = self.flags | (_lib.PTX_FLAG_ACCUMULATE if accumulate else 0)
+        if self._ws_bytes() > self.ws.numel():
+            self.ws = _lib.workspace(self._ws_bytes(), self.device)
         _lib.call(
             "ptx_nearfield_step",
             _lib.ptr(sample), self.H, self.W, _lib.ptr(probe), self.h, self.w, _lib.ptr(transfer),
