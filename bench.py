@@ -46,6 +46,8 @@ def parse_args(argv=None):
                     help="object gradient by float atomics instead of the (default) sorted-segment gather")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in FFT schedule (skip the autotuner)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 256 MiB float32 scale-copy kernel (calibrates HBM byte counters)")
     ap.add_argument("--no-modalities", action="store_true",
                     help="skip the Fourier / near-field / coded full-size measurements")
     return ap.parse_args(argv)
@@ -331,6 +333,14 @@ def main(argv=None) -> int:
                          f"(512x512 probe on the 2048x2048 object), fwd + MSE + adjoint, FP32 torch CPU, "
                          f"per-view Python loops as in the reference, {t_cpu:.1f} s"}
         print(f"[bench] cpu baseline: {cpu['value']} patterns/s on {threads} threads", file=sys.stderr, flush=True)
+
+    if args.copy_probe:
+        # a plain streaming kernel of known size: reads 256 MiB, writes 256 MiB
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
 
     modalities = None if args.no_modalities else measure_modalities(dev, args)
 
