@@ -149,6 +149,36 @@ ptx_status ptx_base_step(const void* sample, int64_t sample_h, int64_t sample_w,
                          void* grad_sample, void* grad_probe,
                          void* workspace, size_t workspace_bytes, unsigned flags, void* stream);
 
+/* Same step for a permuted mini-batch, without copying data: view b of the
+ * batch is entry order[b] of the full tables positions_all [count][2] and
+ * measured_all [count, h, w] (python/train_ptychography.py:532-553 draws
+ * torch.randperm each epoch and indexes both tables).  An order entry outside
+ * [0, count) reads entry 0 instead.  Workspace: ptx_base_step_workspace. */
+ptx_status ptx_base_step_indexed(const void* sample, int64_t sample_h, int64_t sample_w,
+                                 const void* probe, int64_t probe_h, int64_t probe_w,
+                                 const int64_t* positions_all, const float* measured_all, int64_t count,
+                                 const int64_t* order, int64_t batch, float loss_scale,
+                                 float* loss_per_position, float* amplitude,
+                                 void* grad_sample, void* grad_probe,
+                                 void* workspace, size_t workspace_bytes, unsigned flags, void* stream);
+
+/* ------------------------------------------------------------------ optimiser
+ * One Adam update with torch.optim.Adam's arithmetic (foreach variant) as the
+ * reference drivers configure it (python/train_ptychography.py:500-507,
+ * python/train_fpm.py, python/train_coded.py): on n floats -- a complex64
+ * tensor is 2n floats updated per real component, as torch does for complex
+ * parameters.  hyper: device float[3] = {lr / (1 - beta1^t), sqrt(1 - beta2^t),
+ * eps}, read when the kernel runs, so a captured graph of a whole epoch is
+ * replayed with new step sizes by rewriting that buffer. */
+ptx_status ptx_adam_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+                         float beta1, float beta2, const float* hyper, void* stream);
+
+/* positions_out[b] = positions_all[order[b]] (int64 pairs); entries of order
+ * outside [0, count) read entry 0 and set *bad_index_flag to 1 (the caller
+ * zeroes the flag). */
+ptx_status ptx_gather_positions(const int64_t* order, const int64_t* positions_all, int64_t count, int64_t batch,
+                                int64_t* positions_out, int* bad_index_flag, void* stream);
+
 /* ------------------------------------------------------------------ Fourier ptychography
  * Forward of ptychoX_fourier: object_fft = fft2(object_field);
  * sub_b[r,c] = object_fft[(y_b + r + H/2) mod H, (x_b + c + W/2) mod W];

@@ -58,6 +58,12 @@ SIGNATURES: dict[str, tuple] = {
         _st,
         [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _f, _p, _p, _p, _p, _p, _sz, _u, _p],
     ),
+    "ptx_base_step_indexed": (
+        _st,
+        [_p, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _p, _i64, _f, _p, _p, _p, _p, _p, _sz, _u, _p],
+    ),
+    "ptx_adam_step": (_st, [_p, _p, _p, _p, _i64, _f, _f, _p, _p]),
+    "ptx_gather_positions": (_st, [_p, _p, _i64, _i64, _p, _p, _p]),
     "ptx_fpm_forward": (_st, [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _p, _p, _p]),
     "ptx_fpm_backward_workspace": (_sz, [_i64, _i64, _i64, _i64, _i64, _u]),
     "ptx_fpm_backward": (
@@ -153,10 +159,26 @@ def workspace(nbytes: int, device: torch.device) -> torch.Tensor:
     return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
 
 
+_HIP_EVENT_DISABLE_SYSTEM_FENCE = 0x20000000  # hip_runtime_api.h
+
+
+def _hip() -> ctypes.CDLL:
+    """The HIP runtime already mapped by torch (same soname, one runtime)."""
+    h = ctypes.CDLL("libamdhip64.so.7")
+    h.hipEventCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+    h.hipEventDestroy.argtypes = [ctypes.c_void_p]
+    h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+    h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+    return h
+
+
 class KernelTimer:
     """Per-kernel durations of the next entry-point calls, from hipEvents the
     library records on the call's stream between its kernels
-    (``ptx_timing_events``).  Use::
+    (``ptx_timing_events``).  The events are created with
+    ``hipEventDisableSystemFence``: a timing-only event then adds no system-scope
+    cache writeback between kernels, which would otherwise slow the next kernel
+    and inflate its measured duration.  Use::
 
         with KernelTimer(n_events=64) as kt:
             step(...)
@@ -164,14 +186,19 @@ class KernelTimer:
     """
 
     def __init__(self, n_events: int = 64):
-        self.events = [torch.cuda.Event(enable_timing=True) for _ in range(n_events)]
-        for e in self.events:  # materialise the hipEvent_t handles
-            e.record()
+        self._h = _hip()
         torch.cuda.synchronize()
-        self._arr = (ctypes.c_void_p * n_events)(*[e.cuda_event for e in self.events])
+        self._ev = []
+        for _ in range(n_events):
+            e = ctypes.c_void_p()
+            if self._h.hipEventCreateWithFlags(ctypes.byref(e), _HIP_EVENT_DISABLE_SYSTEM_FENCE) != 0:
+                raise NativeLibraryError("hipEventCreateWithFlags failed")
+            self._ev.append(e)
+        self._arr = (ctypes.c_void_p * n_events)(*[e.value for e in self._ev])
+        self.count, self.labels = 0, []
 
     def __enter__(self):
-        call("ptx_timing_events", ctypes.cast(self._arr, ctypes.c_void_p), len(self.events))
+        call("ptx_timing_events", ctypes.cast(self._arr, ctypes.c_void_p), len(self._ev))
         return self
 
     def __exit__(self, *exc):
@@ -181,8 +208,15 @@ class KernelTimer:
         return False
 
     def durations(self) -> list[tuple[str, float]]:
-        torch.cuda.synchronize()
-        return [
-            (self.labels[k], self.events[k - 1].elapsed_time(self.events[k]))
-            for k in range(1, self.count)
-        ]
+        if self.count:
+            self._h.hipEventSynchronize(self._ev[self.count - 1])
+        out = []
+        for k in range(1, self.count):
+            ms = ctypes.c_float()
+            self._h.hipEventElapsedTime(ctypes.byref(ms), self._ev[k - 1], self._ev[k])
+            out.append((self.labels[k], float(ms.value)))
+        return out
+
+    def __del__(self):
+        for e in getattr(self, "_ev", []):
+            self._h.hipEventDestroy(e)

@@ -75,8 +75,9 @@ struct BaseGammaCols {
 // ---------------------------------------------------------------- C12: fused forward/loss/adjoint column pass
 template <int N, int E, int C>
 __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
-    k_base_step_cols(float2* __restrict__ tb, const float* __restrict__ measured, float* __restrict__ amp_out,
-                     float* __restrict__ loss_part, int64_t width, int64_t tiles_per_frame, float gscale) {
+    k_base_step_cols(float2* __restrict__ tb, const float* __restrict__ measured, const int64_t* __restrict__ order,
+                     int64_t count, float* __restrict__ amp_out, float* __restrict__ loss_part, int64_t width,
+                     int64_t tiles_per_frame, float gscale) {
   using Cf = ColCfg<N, E>;
   constexpr int T = Cf::T;
   __shared__ float2 lds[C * lds_len<N>()];
@@ -87,6 +88,13 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   const int64_t tile = blockIdx.x % tiles_per_frame;
   const int64_t col = tile * C + c;
   const int64_t off = frame * N * width;  // frame base (uniform)
+  // measured frame of this view: the batch's own (order == null) or measured[order[frame]]
+  int64_t mframe = frame;
+  if (order) {
+    mframe = order[frame];
+    if (mframe < 0 || mframe >= count) mframe = 0;  // flagged by the position gather
+  }
+  const float* meas = measured + mframe * N * width;
   float2 v[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = tb[off + ((t + q * T) * (int)width + (int)col)];
@@ -96,7 +104,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   for (int q = 0; q < E; ++q) {
     const int64_t k = off + ((t + q * T) * (int)width + (int)col);
     const float a = c_abs(v[q]);
-    const float d = a - measured[k];
+    const float d = a - meas[k - off];
     lsum += d * d;
     if (amp_out) amp_out[k] = a;
     v[q] = c_scale(v[q], gscale * d * fast_rcp(fmaxf(a, kFltEps)));
@@ -128,18 +136,18 @@ struct GammaColsRun {
 
 template <int N, int E>
 struct StepColsRun {
-  static ptx_status run(float2* tb, const float* measured, float* amp, float* loss_part, int64_t frames, int64_t width,
-                        float gscale, hipStream_t s) {
+  static ptx_status run(float2* tb, const float* measured, const int64_t* order, int64_t count, float* amp,
+                        float* loss_part, int64_t frames, int64_t width, float gscale, hipStream_t s) {
     using Cf = ColCfg<N, E>;
     if (width >= Cf::CMAX) {
       constexpr int C = Cf::CMAX;
       const int64_t tiles = width / C;
-      k_base_step_cols<N, E, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(tb, measured, amp, loss_part,
+      k_base_step_cols<N, E, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(tb, measured, order, count, amp, loss_part,
                                                                                             width, tiles, gscale);
     } else {
       constexpr int C = 8;
       const int64_t tiles = width / C;
-      k_base_step_cols<N, E, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(tb, measured, amp, loss_part,
+      k_base_step_cols<N, E, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(tb, measured, order, count, amp, loss_part,
                                                                                             width, tiles, gscale);
     }
     return check_launch("base step cols", s);
@@ -220,18 +228,26 @@ size_t ptx_base_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64
   dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
   int64_t tiles = 0;
   dispatch_ne<ColTilesQuery>(PASS_COL, h, w, &tiles);
-  return carve_scan(nullptr, H, W, h, w, B, flags, true, sp, tiles).bytes;
+  // room for the indexed variant's gathered positions too
+  return carve_scan(nullptr, H, W, h, w, B, flags, true, sp, tiles, true).bytes;
 }
 
-ptx_status ptx_base_step(const void* sample, int64_t H, int64_t W, const void* probe, int64_t h, int64_t w,
-                         const int64_t* pos, int64_t B, const float* measured, float loss_scale,
-                         float* loss_per_position, float* amplitude, void* grad_sample, void* grad_probe,
-                         void* workspace, size_t ws_bytes, unsigned flags, void* stream) {
-  ptx_status st = check_shapes("ptx_base_step", H, W, h, w, B);
+}  // extern "C"
+
+namespace ptx {
+namespace {
+
+// Shared body of ptx_base_step and ptx_base_step_indexed.  order == null: the
+// batch's own positions / measured frames; otherwise view b is order[b] of the
+// full tables positions_all[count] / measured_all[count].
+ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t W, const void* probe, int64_t h,
+                          int64_t w, const int64_t* pos, int64_t count, const float* measured, const int64_t* order,
+                          int64_t B, float loss_scale, float* loss_per_position, float* amplitude, void* grad_sample,
+                          void* grad_probe, void* workspace, size_t ws_bytes, unsigned flags, hipStream_t s) {
+  ptx_status st = check_shapes(fn, H, W, h, w, B);
   if (st != PTX_OK) return st;
-  hipStream_t s = (hipStream_t)stream;
   if (!grad_sample || !grad_probe) {
-    set_error("ptx_base_step: null gradient output");
+    set_error("%s: null gradient output", fn);
     return PTX_ERR_INVALID_ARGUMENT;
   }
   if (B == 0) {
@@ -242,28 +258,63 @@ ptx_status ptx_base_step(const void* sample, int64_t H, int64_t W, const void* p
     return check_launch("zero fill", s);
   }
   if (!sample || !probe || !pos || !measured || !loss_per_position) {
-    set_error("ptx_base_step: null pointer argument");
+    set_error("%s: null pointer argument", fn);
     return PTX_ERR_INVALID_ARGUMENT;
   }
   ScatterPlan sp{};
   dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
   int64_t tiles = 0;
   dispatch_ne<ColTilesQuery>(PASS_COL, h, w, &tiles);
-  const ScanWorkspace ws = carve_scan(workspace, H, W, h, w, B, flags, true, sp, tiles);
+  const ScanWorkspace ws = carve_scan(workspace, H, W, h, w, B, flags, true, sp, tiles, order != nullptr);
   if (!workspace || ws_bytes < ws.bytes) {
-    set_error("ptx_base_step: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
+    set_error("%s: workspace of %zu bytes needed, %zu given", fn, ws.bytes, ws_bytes);
     return PTX_ERR_WORKSPACE;
   }
   timing_begin(s);
-  ScanGatherRows<true> r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, pos, ws.tb};
+  const int64_t* bpos = pos;
+  if (order) {
+    fill_zero2(reinterpret_cast<float2*>(ws.bad_index), 1, s);
+    if ((st = launch_gather_positions(order, pos, count, B, ws.pos_batch, ws.bad_index, s)) != PTX_OK) return st;
+    bpos = ws.pos_batch;
+  }
+  ScanGatherRows<true> r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, bpos, ws.tb};
   st = dispatch_ne<GatherRowsRun>(PASS_ROW, w, r1, B * h, s);
   if (st != PTX_OK) return st;
-  st = dispatch_ne<StepColsRun>(PASS_COL, h, ws.tb, measured, amplitude, ws.loss_part, B, w, 2.0f * loss_scale, s);
+  st = dispatch_ne<StepColsRun>(PASS_COL, h, ws.tb, measured, order, count, amplitude, ws.loss_part, B, w,
+                                2.0f * loss_scale, s);
   if (st != PTX_OK) return st;
   loss_rows(ws.loss_part, tiles, B, 1.0f / (float)(h * w), loss_per_position, s);
   if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
-  return scan_scatter<true>(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, pos, B,
+  return scan_scatter<true>(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, bpos, B,
                             (float2*)grad_sample, (float2*)grad_probe, flags, 1.0f, s);
+}
+
+}  // namespace
+}  // namespace ptx
+
+extern "C" {
+
+ptx_status ptx_base_step(const void* sample, int64_t H, int64_t W, const void* probe, int64_t h, int64_t w,
+                         const int64_t* pos, int64_t B, const float* measured, float loss_scale,
+                         float* loss_per_position, float* amplitude, void* grad_sample, void* grad_probe,
+                         void* workspace, size_t ws_bytes, unsigned flags, void* stream) {
+  return base_step_impl("ptx_base_step", sample, H, W, probe, h, w, pos, B, measured, nullptr, B, loss_scale,
+                        loss_per_position, amplitude, grad_sample, grad_probe, workspace, ws_bytes, flags,
+                        (hipStream_t)stream);
+}
+
+ptx_status ptx_base_step_indexed(const void* sample, int64_t H, int64_t W, const void* probe, int64_t h, int64_t w,
+                                 const int64_t* positions_all, const float* measured_all, int64_t count,
+                                 const int64_t* order, int64_t B, float loss_scale, float* loss_per_position,
+                                 float* amplitude, void* grad_sample, void* grad_probe, void* workspace,
+                                 size_t ws_bytes, unsigned flags, void* stream) {
+  if (B > 0 && !order) {
+    set_error("ptx_base_step_indexed: null view order");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  return base_step_impl("ptx_base_step_indexed", sample, H, W, probe, h, w, positions_all, count, measured_all,
+                        order, B, loss_scale, loss_per_position, amplitude, grad_sample, grad_probe, workspace,
+                        ws_bytes, flags, (hipStream_t)stream);
 }
 
 }  // extern "C"

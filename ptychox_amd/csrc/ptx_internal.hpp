@@ -29,6 +29,11 @@ inline bool size_ok(int64_t n) { return is_pow2(n) && n >= PTX_MIN_N && n <= PTX
 // kernel(s) named `what`.
 ptx_status check_launch(const char* what, hipStream_t stream = nullptr);
 
+// positions_out[b] = positions_all[order[b]]; an order entry outside [0, count)
+// reads entry 0 instead and sets *bad (defined in ptx_optim.hip).
+ptx_status launch_gather_positions(const int64_t* order, const int64_t* positions_all, int64_t count, int64_t batch,
+                                   int64_t* positions_out, int* bad, hipStream_t stream);
+
 // Records timing event 0 when armed; called by each entry point before its first kernel.
 void timing_begin(hipStream_t stream);
 

@@ -165,11 +165,13 @@ struct ScanWorkspace {
   float* loss_part;  // [B, col tiles]   (step only)
   int* lists;        // [ntiles, B]       (deterministic only)
   int* counts;       // [ntiles]
+  int64_t* pos_batch;  // [B, 2] gathered positions (indexed step only)
+  int* bad_index;      // set when an order entry is out of range (indexed step only)
   size_t bytes;
 };
 
 ScanWorkspace carve_scan(void* base, int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags,
-                         bool step, const ScatterPlan& sp, int64_t col_tiles) {
+                         bool step, const ScatterPlan& sp, int64_t col_tiles, bool indexed = false) {
   Carver cv{(char*)base, 0};
   ScanWorkspace ws{};
   ws.tb = cv.take<float2>((size_t)(B * h * w));
@@ -179,6 +181,10 @@ ScanWorkspace carve_scan(void* base, int64_t H, int64_t W, int64_t h, int64_t w,
     const TileGrid tg = tile_grid(H, W);
     ws.lists = cv.take<int>((size_t)(tg.count() * B));
     ws.counts = cv.take<int>((size_t)tg.count());
+  }
+  if (indexed) {
+    ws.pos_batch = cv.take<int64_t>((size_t)(2 * B));
+    ws.bad_index = reinterpret_cast<int*>(cv.take<float2>(1));  // zeroed with the float2 fill kernel
   }
   ws.bytes = cv.used;
   return ws;
