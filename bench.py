@@ -48,6 +48,7 @@ def parse_args(argv=None):
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in FFT schedule (skip the autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB float32 scale-copy kernel (calibrates HBM byte counters)")
+    ap.add_argument("--no-epoch", action="store_true", help="skip the whole-epoch (Adam, hipGraph) measurement")
     ap.add_argument("--no-modalities", action="store_true",
                     help="skip the Fourier / near-field / coded full-size measurements")
     return ap.parse_args(argv)
@@ -100,6 +101,65 @@ def _timed(fn, reps):
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def measure_epoch(ds, dev) -> dict:
+    """Whole training epochs of the reference driver's loop (permutation,
+    900 positions in mini-batches of 64 with a partial last batch, MSE loss,
+    Adam on object and probe): the device-resident loop captured as a hipGraph,
+    the same loop launched eagerly, and the plain torch loop."""
+    import torch
+
+    from ptychox_amd import ptychoX_base
+    from ptychox_amd.train import ConventionalReconstruction
+
+    P = ds.positions.shape[0]
+    init_s = torch.ones_like(ds.sample)
+    init_p = (ds.probe.abs() > 0).to(torch.complex64)
+    out = {"positions": P, "mini_batch": BATCH}
+    for name, graph in (("graph", True), ("eager", False)):
+        gen = torch.Generator(device=dev).manual_seed(0)
+        rec = ConventionalReconstruction(init_s, init_p, ds.positions, ds.measured, BATCH, lr=1e-2, epochs=10,
+                                         generator=gen, graph=graph)
+        rec.run_epoch(1)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        n = 5
+        for e in range(2, 2 + n):
+            rec.run_epoch(e)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / n
+        out[name] = {"ms_per_epoch": round(ms, 3), "patterns_per_s": round(P / (ms / 1e3), 1)}
+    s = torch.nn.Parameter(init_s.clone())
+    p = torch.nn.Parameter(init_p.clone())
+    opt = torch.optim.Adam([s, p], lr=1e-2, betas=(0.9, 0.999), eps=1e-16)
+
+    def torch_epoch():
+        order = torch.randperm(P, device=dev)
+        for start in range(0, P, BATCH):
+            bi = order[start : start + BATCH]
+            loss = torch.nn.functional.mse_loss(ptychoX_base(s, p, ds.positions[bi]), ds.measured[bi])
+            opt.zero_grad(set_to_none=True)
+            loss.backward()
+            opt.step()
+
+    torch_epoch()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    torch_epoch()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    out["plain_torch"] = {"ms_per_epoch": round(ms, 3), "patterns_per_s": round(P / (ms / 1e3), 1)}
+    out["graph_speedup_vs_plain"] = round(ms / out["graph"]["ms_per_epoch"], 2)
+    print(f"[bench] epoch (900 positions, Adam): graph {out['graph']['ms_per_epoch']} ms, eager "
+          f"{out['eager']['ms_per_epoch']} ms, plain torch {ms:.3f} ms", file=sys.stderr, flush=True)
+    return out
 
 
 def measure_modalities(dev, args) -> dict:
@@ -342,6 +402,7 @@ def main(argv=None) -> int:
         torch.cuda.synchronize()
         del a, b
 
+    epoch = None if args.no_epoch else measure_epoch(ds, dev)
     modalities = None if args.no_modalities else measure_modalities(dev, args)
 
     out = {
@@ -370,6 +431,7 @@ def main(argv=None) -> int:
         "tuning": tuning,
         "plain_path": plain,
         "cpu_baseline": cpu,
+        "epoch": epoch,
         "other_workloads": modalities,
     }
     print(json.dumps(out), flush=True)
