@@ -49,6 +49,8 @@ def parse_args(argv=None):
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB float32 scale-copy kernel (calibrates HBM byte counters)")
     ap.add_argument("--no-epoch", action="store_true", help="skip the whole-epoch (Adam, hipGraph) measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-requests", type=int, default=40, help="requests in the serving measurement")
     ap.add_argument("--no-modalities", action="store_true",
                     help="skip the Fourier / near-field / coded full-size measurements")
     return ap.parse_args(argv)
@@ -239,6 +241,82 @@ def measure_modalities(dev, args) -> dict:
     return out
 
 
+def measure_serving(dev, args) -> dict:
+    """The serving loop on the mixed request stream: conventional probe 128 /
+    object 512 and probe 256 / object 1024 (100-400 positions), Fourier 128^2
+    pupil / 512^2 object (81 or 225 LEDs), near-field 512 / 1024 (64
+    positions), coded 512^2 (25 translations); 20 Adam epochs per request.
+
+    * saturation: every request arrives at t = 0; requests/s of the server and
+      of the plain per-request path (one request at a time, unfused torch ops
+      + mse_loss + backward + torch.optim.Adam, after a warm pass so rocFFT
+      plans exist);
+    * latency: p50 / p99 from arrival to completion under Poisson arrivals at
+      three rates (the plain path at half its own capacity, the server there
+      and at 0.5x and 0.9x its saturation rate)."""
+    import math
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    from ptychox_amd import serving as sv
+
+    n = args.serve_requests
+    base = sv.standard_mix(n, math.inf, seed=5, epochs=20)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(8) as ex:
+        probs = dict(zip([r.rid for r in base], ex.map(sv.make_problem, base)))
+    print(f"[bench] serving: {n} requests synthesised in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+
+    def arrivals(rate, seed):
+        rng = np.random.default_rng(seed)
+        t = np.cumsum(rng.exponential(1.0 / rate, n))
+        return [sv.Request(float(t[i]), r.rid, r.kind, r.size, r.views, r.seed, r.epochs, r.lr)
+                for i, r in enumerate(base)]
+
+    sv.Server(dev).run(base[:10], probs)  # warm-up: first launches, graph machinery
+    srv = sv.Server(dev)
+    sat = sv.summarize(srv.run(base, probs))
+    sat["peak_in_flight"] = srv.peak_in_flight
+    sat["peak_resident_gb"] = round(srv.peak_bytes / 1e9, 3)
+    print(f"[bench] serving saturation: {sat}", file=sys.stderr, flush=True)
+
+    sub = base[: min(10, n)]
+    sv.run_plain(sub, probs, dev)  # warm pass: rocFFT plans for every shape
+    plain_done = sv.run_plain(sub, probs, dev)
+    plain_busy = sum(c.finished - c.admitted for c in plain_done)
+    plain_rps = len(sub) / plain_busy
+    plain_sat = {"requests": len(sub), "throughput_req_per_s": round(plain_rps, 3),
+                 "mean_service_ms": round(1e3 * plain_busy / len(sub), 2)}
+    print(f"[bench] plain per-request path: {plain_sat}", file=sys.stderr, flush=True)
+
+    rates = []
+    low = 0.5 * plain_rps
+    low_reqs = [r for r in arrivals(low, 11)[: len(sub)]]
+    rates.append({"rate_req_per_s": round(low, 3),
+                  "server": sv.summarize(sv.Server(dev).run(low_reqs, probs)),
+                  "plain": sv.summarize(sv.run_plain(low_reqs, probs, dev))})
+    for frac, seed in ((0.5, 12), (0.9, 13)):
+        rate = frac * sat["throughput_req_per_s"]
+        rates.append({"rate_req_per_s": round(rate, 3), "server": sv.summarize(sv.Server(dev).run(arrivals(rate, seed),
+                                                                                              probs)),
+                      "plain": None})
+    for r in rates:
+        print(f"[bench] serving at {r['rate_req_per_s']} req/s: server {r['server']}"
+              + (f", plain {r['plain']}" if r["plain"] else ""), file=sys.stderr, flush=True)
+    return {
+        "workload": "mixed stream: conventional 128/512 and 256/1024 (100-400 positions), Fourier 128^2/512^2 "
+                    "(81|225 LEDs), near-field 512/1024 (64), coded 512^2 (25); 20 Adam epochs each",
+        "requests": n,
+        "saturation": {"server": sat, "plain": plain_sat,
+                       "speedup": round(sat["throughput_req_per_s"] / plain_rps, 1)},
+        "poisson": rates,
+        "plain_what": "one request at a time: ptychoX_*(use_cuda_kernels=False) + F.mse_loss + backward + "
+                      "torch.optim.Adam (eps 1e-16), coded surface projected as the reference driver does",
+    }
+
+
 def main(argv=None) -> int:
     args = parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -404,6 +482,7 @@ def main(argv=None) -> int:
 
     epoch = None if args.no_epoch else measure_epoch(ds, dev)
     modalities = None if args.no_modalities else measure_modalities(dev, args)
+    serving = None if args.no_serve else measure_serving(dev, args)
 
     out = {
         "metric": METRIC,
@@ -433,6 +512,7 @@ def main(argv=None) -> int:
         "cpu_baseline": cpu,
         "epoch": epoch,
         "other_workloads": modalities,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     return 0

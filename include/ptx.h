@@ -169,9 +169,14 @@ ptx_status ptx_base_step_indexed(const void* sample, int64_t sample_h, int64_t s
  * tensor is 2n floats updated per real component, as torch does for complex
  * parameters.  hyper: device float[3] = {lr / (1 - beta1^t), sqrt(1 - beta2^t),
  * eps}, read when the kernel runs, so a captured graph of a whole epoch is
- * replayed with new step sizes by rewriting that buffer. */
+ * replayed with new step sizes by rewriting that buffer.
+ * PTX_ADAM_PROJECT_REAL_NONNEG (param is complex64, n even) follows the update
+ * with the coded driver's constraint on the coded surface,
+ * param = complex(max(Re param, 0), 0) (python/train_coded.py:240-241); the
+ * moments of both components are updated as torch does before that copy. */
+#define PTX_ADAM_PROJECT_REAL_NONNEG 1u
 ptx_status ptx_adam_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
-                         float beta1, float beta2, const float* hyper, void* stream);
+                         float beta1, float beta2, const float* hyper, unsigned flags, void* stream);
 
 /* positions_out[b] = positions_all[order[b]] (int64 pairs); entries of order
  * outside [0, count) read entry 0 and set *bad_index_flag to 1 (the caller

@@ -18,6 +18,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libptychox_hip.so"
 PTX_FLAG_DETERMINISTIC = 1
 PTX_FLAG_ACCUMULATE = 2
 PTX_FLAG_ATOMIC_SCATTER = 4
+PTX_ADAM_PROJECT_REAL_NONNEG = 1
 
 _STATUS = {
     0: "PTX_OK",
@@ -62,7 +63,7 @@ SIGNATURES: dict[str, tuple] = {
         _st,
         [_p, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _p, _i64, _f, _p, _p, _p, _p, _p, _sz, _u, _p],
     ),
-    "ptx_adam_step": (_st, [_p, _p, _p, _p, _i64, _f, _f, _p, _p]),
+    "ptx_adam_step": (_st, [_p, _p, _p, _p, _i64, _f, _f, _p, _u, _p]),
     "ptx_gather_positions": (_st, [_p, _p, _i64, _i64, _p, _p, _p]),
     "ptx_fpm_forward": (_st, [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _p, _p, _p]),
     "ptx_fpm_backward_workspace": (_sz, [_i64, _i64, _i64, _i64, _i64, _u]),

@@ -16,6 +16,9 @@ namespace {
 // torch's foreach Adam (torch/optim/adam.py, _multi_tensor_adam) in one pass:
 //   m = lerp(m, g, 1 - b1);  v = b2 v + (1 - b2) g^2
 //   denom = sqrt(v) / sqrt(1 - b2^t) + eps;  p += -(lr / (1 - b1^t)) * (m / denom)
+// PROJECT: after the update, interleaved complex pairs become (max(re, 0), 0)
+// with clamp_min's NaN-propagating comparison (x < 0 ? 0 : x).
+template <bool PROJECT>
 static __global__ void __launch_bounds__(256)
     k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v, int64_t n,
            float beta1, float beta2, const float* __restrict__ hyper) {
@@ -31,7 +34,9 @@ static __global__ void __launch_bounds__(256)
     m[i] = mi;
     v[i] = vi;
     const float denom = sqrtf(vi) / bc2_sqrt + eps;
-    p[i] = p[i] + (-step_size) * (mi / denom);
+    float pi = p[i] + (-step_size) * (mi / denom);
+    if (PROJECT) pi = (i & 1) ? 0.0f : (pi < 0.0f ? 0.0f : pi);
+    p[i] = pi;
   }
 }
 
@@ -66,7 +71,7 @@ using namespace ptx;
 extern "C" {
 
 ptx_status ptx_adam_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n, float beta1,
-                         float beta2, const float* hyper, void* stream) {
+                         float beta2, const float* hyper, unsigned flags, void* stream) {
   if (n < 0 || (n > 0 && (!param || !grad || !exp_avg || !exp_avg_sq || !hyper))) {
     set_error("ptx_adam_step: invalid arguments");
     return PTX_ERR_INVALID_ARGUMENT;
@@ -75,12 +80,25 @@ ptx_status ptx_adam_step(float* param, const float* grad, float* exp_avg, float*
     set_error("ptx_adam_step: betas must lie in [0, 1)");
     return PTX_ERR_INVALID_ARGUMENT;
   }
+  if (flags & ~PTX_ADAM_PROJECT_REAL_NONNEG) {
+    set_error("ptx_adam_step: unknown flags");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  const bool project = (flags & PTX_ADAM_PROJECT_REAL_NONNEG) != 0;
+  if (project && (n & 1)) {
+    set_error("ptx_adam_step: the real non-negative projection needs complex pairs (n even)");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
   if (n == 0) return PTX_OK;
   hipStream_t s = (hipStream_t)stream;
   timing_begin(s);
   int64_t blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  k_adam<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(param, grad, exp_avg, exp_avg_sq, n, beta1, beta2, hyper);
+  if (project)
+    k_adam<true><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(param, grad, exp_avg, exp_avg_sq, n, beta1, beta2, hyper);
+  else
+    k_adam<false><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(param, grad, exp_avg, exp_avg_sq, n, beta1, beta2,
+                                                               hyper);
   return check_launch("adam", s);
 }
 

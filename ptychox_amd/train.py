@@ -41,20 +41,25 @@ def adam_hyper(lr: float, beta1: float, beta2: float, eps: float, step: int) -> 
 
 
 class AdamState:
-    """First and second moments of one complex parameter (per real component)."""
+    """First and second moments of one complex parameter (per real component).
 
-    def __init__(self, param: torch.Tensor, beta1: float = 0.9, beta2: float = 0.999):
+    ``real_nonneg=True`` follows every update with the coded driver's
+    constraint on the coded surface, ``p = max(Re p, 0) + 0j``
+    (python/train_coded.py:240-241), in the same kernel."""
+
+    def __init__(self, param: torch.Tensor, beta1: float = 0.9, beta2: float = 0.999, real_nonneg: bool = False):
         self.param = param
         self.m = torch.zeros_like(torch.view_as_real(param))
         self.v = torch.zeros_like(self.m)
         self.beta1, self.beta2 = float(beta1), float(beta2)
+        self.flags = _lib.PTX_ADAM_PROJECT_REAL_NONNEG if real_nonneg else 0
 
     def step(self, grad: torch.Tensor, hyper: torch.Tensor) -> None:
         """One update; ``hyper`` is a device float[3] (see :func:`adam_hyper`)."""
         _lib.call(
             "ptx_adam_step",
             _lib.ptr(self.param), _lib.ptr(grad), _lib.ptr(self.m), _lib.ptr(self.v), self.m.numel(),
-            self.beta1, self.beta2, _lib.ptr(hyper), _lib.stream_of(self.param.device),
+            self.beta1, self.beta2, _lib.ptr(hyper), self.flags, _lib.stream_of(self.param.device),
         )
 
 

@@ -84,6 +84,19 @@ def test_call_raises_with_message(lib):
         lib.call("ptx_base_forward", None, 64, 64, None, 16, 16, None, 1, None, None, None)
 
 
+def test_adam_argument_checks(lib):
+    # every rejection happens on the host before anything is launched
+    fake = ctypes.c_void_p(256)
+    args = (fake, fake, fake, fake)
+    with pytest.raises(ValueError, match="unknown flags"):
+        lib.call("ptx_adam_step", *args, 8, 0.9, 0.999, fake, 6, None)
+    with pytest.raises(ValueError, match="complex pairs"):
+        lib.call("ptx_adam_step", *args, 7, 0.9, 0.999, fake, lib.PTX_ADAM_PROJECT_REAL_NONNEG, None)
+    with pytest.raises(ValueError, match="betas"):
+        lib.call("ptx_adam_step", *args, 8, 1.0, 0.999, fake, 0, None)
+    lib.call("ptx_adam_step", *args, 0, 0.9, 0.999, fake, 0, None)  # n = 0: nothing to do
+
+
 def test_timing_hook_arms_and_disarms(lib):
     lib.call("ptx_timing_events", None, 0)
     assert lib.lib().ptx_timing_count() == 0

@@ -36,6 +36,35 @@ def test_adam_matches_torch(gpu):
     assert e < 1e-6
 
 
+def test_adam_projection_matches_coded_driver(gpu):
+    # torch Adam followed by the coded driver's copy_(real.clamp_min(0)) on the
+    # coded surface (python/train_coded.py:240-241)
+    from ptychox_amd.train import AdamState, adam_hyper
+
+    torch.manual_seed(1)
+    p0 = torch.randn(130, 66, dtype=torch.complex64, device=gpu)
+    ours = p0.clone()
+    ref = torch.nn.Parameter(p0.clone())
+    opt = torch.optim.Adam([ref], lr=5e-2, betas=(0.9, 0.999), eps=1e-16)
+    st = AdamState(ours, real_nonneg=True)
+    hyper = torch.zeros(4, device=gpu)
+    for t in range(1, 6):
+        g = torch.randn_like(p0)
+        ref.grad = g.clone()
+        opt.step()
+        with torch.no_grad():
+            ref.copy_(ref.real.clamp_min(0.0).to(torch.complex64))
+        hyper[:3] = torch.tensor(adam_hyper(5e-2, 0.9, 0.999, 1e-16, t), device=gpu)
+        st.step(g, hyper)
+    torch.cuda.synchronize()
+    assert torch.all(ours.imag == 0) and torch.all(ours.real >= 0)
+    assert (ours.real == 0).float().mean() > 0.2  # the constraint is active
+    e = nrel(ours, ref.detach())
+    print(f"projected adam vs torch after 5 steps: {e:.2e}", flush=True)
+    # single-ulp differences of the update (torch's foreach op order) at lr 5e-2
+    assert e < 5e-6
+
+
 def _problem(gpu, seed=0):
     rng = np.random.default_rng(seed)
     truth = smooth_object(rng, 192, 3.0)
