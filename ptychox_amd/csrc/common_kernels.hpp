@@ -58,10 +58,10 @@ __device__ __forceinline__ float block_sum(float x, float* red /* >= 16 floats o
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nwaves = (blockDim.x + 63) >> 6;
   if (lane == 0) red[wave] = x;
-  __syncthreads();
+  lds_barrier();
   float s = 0.0f;
   for (int w = 0; w < nwaves; ++w) s += red[w];  // fixed order
-  __syncthreads();
+  lds_barrier();
   return s;
 }
 

@@ -173,13 +173,27 @@ struct FftShape {
   static constexpr bool NEEDS_LDS = STAGES > 1;
 };
 
-template <int N, int E, int DIR, int R, int NS, bool LAST, bool SYNC_FIRST>
-__device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restrict__ lds) {
+// Work-group barrier that orders LDS accesses only: waits for this wave's LDS
+// operations (lgkmcnt) but not for its global loads and stores, so global
+// loads issued ahead of the transform (prefetch of the next tile) stay in
+// flight across the exchanges.  __syncthreads() would also drain vmcnt.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// LDS_TW: the stage's base twiddles come from `lds_tw`, an LDS copy of
+// w_N^k (k < N, forward direction; see load_lds_twiddles) instead of the
+// global table.
+template <int N, int E, int DIR, int R, int NS, bool LAST, bool SYNC_FIRST, bool LDS_TW = false>
+__device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restrict__ lds,
+                                          const float2* __restrict__ lds_tw = nullptr) {
   constexpr int T = N / E;
   constexpr int NB = E / R;               // butterflies per thread
   constexpr int TWS = TW_MAX / (NS * R);  // twiddle stride into the table
   // single-buffer mode: everyone must have finished reading the previous exchange
-  if constexpr (SYNC_FIRST) __syncthreads();
+  if constexpr (SYNC_FIRST) lds_barrier();
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     const int j = t + i * T;
@@ -190,7 +204,13 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restr
     if constexpr (NS > 1) {
       // w^(k m) by recurrence from one table load: relative error grows by
       // ~1 ulp per step (< 1e-6 for R = 16) and saves R - 2 memory loads
-      const float2 w1 = twiddle<DIR>(k * TWS);
+      float2 w1;
+      if constexpr (LDS_TW) {
+        const float2 w = lds_tw[k * (N / (NS * R))];
+        w1 = DIR > 0 ? make_float2(w.x, -w.y) : w;
+      } else {
+        w1 = twiddle<DIR>(k * TWS);
+      }
       float2 wm = w1;
 #pragma unroll
       for (int m = 1; m < R; ++m) {
@@ -209,24 +229,37 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restr
     }
   }
   if constexpr (!LAST) {
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = lds[lds_pad(t + q * T)];
   }
 }
 
+// LDS copy of the forward twiddles w_N^k, k < N (N float2), written by the
+// whole work-group; a kernel that transforms many sequences reads its stage
+// twiddles from here, so its sequence loop issues no global twiddle loads --
+// which matters when the loop also keeps global loads of the next sequence in
+// flight (vector-memory counters retire in order, so waiting for a later
+// twiddle load would also wait for those).  Ends with a barrier.
+template <int N>
+__device__ __forceinline__ void load_lds_twiddles(float2* lds_tw) {
+  for (int k = threadIdx.x; k < N; k += blockDim.x) lds_tw[k] = twiddle<-1>(k * (TW_MAX / N));
+  lds_barrier();
+}
+
 // PINGPONG: alternate two LDS regions (one barrier per exchange); otherwise a
 // single region and two barriers per exchange (half the LDS footprint).
-template <int N, int E, int DIR, bool PINGPONG, int S>
-__device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, float2* lds0, float2* lds1) {
+template <int N, int E, int DIR, bool PINGPONG, int S, bool LDS_TW = false>
+__device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, float2* lds0, float2* lds1,
+                                           const float2* lds_tw = nullptr) {
   using Sh = FftShape<N, E>;
   if constexpr (S < Sh::STAGES) {
     constexpr int R = S < Sh::FULL ? E : Sh::REM;
     constexpr int NS = ipow(E, S);
     constexpr bool LAST = S == Sh::STAGES - 1;
     constexpr bool SYNC_FIRST = !PINGPONG && S > 0 && !LAST;
-    fft_stage<N, E, DIR, R, NS, LAST, SYNC_FIRST>(v, t, (PINGPONG && (S & 1)) ? lds1 : lds0);
-    fft_stages<N, E, DIR, PINGPONG, S + 1>(v, t, lds0, lds1);
+    fft_stage<N, E, DIR, R, NS, LAST, SYNC_FIRST, LDS_TW>(v, t, (PINGPONG && (S & 1)) ? lds1 : lds0, lds_tw);
+    fft_stages<N, E, DIR, PINGPONG, S + 1, LDS_TW>(v, t, lds0, lds1, lds_tw);
   }
 }
 
@@ -237,6 +270,13 @@ __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, float2* lds0, 
 template <int N, int E, int DIR, bool PINGPONG = false>
 __device__ __forceinline__ void fft_seq(float2 (&v)[E], int t, float2* lds0, float2* lds1 = nullptr) {
   fft_stages<N, E, DIR, PINGPONG, 0>(v, t, lds0, lds1);
+}
+
+// The same transform with the stage twiddles read from an LDS table
+// (load_lds_twiddles<N>).
+template <int N, int E, int DIR>
+__device__ __forceinline__ void fft_seq_tw(float2 (&v)[E], int t, float2* lds, const float2* lds_tw) {
+  fft_stages<N, E, DIR, false, 0, true>(v, t, lds, nullptr, lds_tw);
 }
 
 // Elements per thread used for a transform of length N.

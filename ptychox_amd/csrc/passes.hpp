@@ -41,6 +41,21 @@ struct ColCfg {
   static constexpr int CMAX = CM0 >= 16 ? 16 : CM0 >= 8 ? 8 : CM0 >= 4 ? 4 : 2;
 };
 
+// Block -> tile order of the column passes.  A column tile of C columns reads
+// C*8 bytes of each complex row (C*4 of a float row), which is half of a
+// 128-byte line when C = 8 (complex) or C = 16 (float measurements); the other
+// half belongs to the neighbouring tile.  Work-groups are dispatched to the 8
+// XCDs round-robin (block b runs on XCD b % 8, each XCD with its own L2), so in
+// every group of 16 blocks, blocks r and r + 8 -- same XCD, dispatched back to
+// back -- take the adjacent tiles 2r' and 2r' + 1 (r' = r mod 8) and share the
+// line through that L2 instead of both fetching it from HBM.  A bijection on
+// each full group of 16; a grid that is not a multiple of 16 keeps the identity.
+__device__ __forceinline__ int64_t xcd_paired_block(unsigned b, unsigned nblocks) {
+  if (nblocks & 15u) return b;
+  const unsigned r = b & 15u;
+  return (int64_t)((b & ~15u) + 2u * (r & 7u) + (r >> 3));
+}
+
 template <int N, int E, int DIR, class Op>
 __global__ void __launch_bounds__(256) k_rows(Op op, int64_t nrows) {
   using Cf = RowCfg<N, E>;
@@ -61,8 +76,9 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C)) k_cols(Op op, int64_t t
   __shared__ float2 lds[C * lds_len<N>()];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
-  const int64_t frame = blockIdx.x / tiles_per_frame;
-  const int64_t col = (blockIdx.x % tiles_per_frame) * C + c;
+  const int64_t blk = xcd_paired_block(blockIdx.x, gridDim.x);
+  const int64_t frame = blk / tiles_per_frame;
+  const int64_t col = (blk % tiles_per_frame) * C + c;
   float2 v[E];
   op.template load<N, E>(frame, col, t, v);
   fft_seq<N, E, DIR>(v, t, &lds[c * lds_len<N>()]);
@@ -106,8 +122,9 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   __shared__ float2 lds[C * lds_len<N>()];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
-  const int64_t frame = blockIdx.x / tiles_per_frame;
-  const int64_t col = (blockIdx.x % tiles_per_frame) * C + c;
+  const int64_t blk = xcd_paired_block(blockIdx.x, gridDim.x);
+  const int64_t frame = blk / tiles_per_frame;
+  const int64_t col = (blk % tiles_per_frame) * C + c;
   float2* p = tb + frame * N * width;
   float2 v[E];
 #pragma unroll
@@ -118,7 +135,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     const float2 tr = transfer[(t + q * T) * (int)width + (int)col];
     v[q] = CONJ ? c_mulc(v[q], tr) : c_mul(v[q], tr);
   }
-  __syncthreads();  // second transform reuses the LDS region
+  lds_barrier();  // second transform reuses the LDS region
   fft_seq<N, E, +1>(v, t, &lds[c * lds_len<N>()]);
 #pragma unroll
   for (int q = 0; q < E; ++q) p[(t + q * T) * (int)width + (int)col] = v[q];

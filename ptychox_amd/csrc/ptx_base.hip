@@ -15,6 +15,7 @@
 //             R2 rows  : row iFFT, sign, chi conj(p) -> object (atomics or per-view buffer),
 //                        chi conj(patch) accumulated over a chunk of views in registers -> probe partials
 //   step:     R1, then C12 cols: column FFT, |Phi|, loss partials, Gamma, column iFFT (in place), then R2.
+#include "buffer_io.hpp"
 #include "scan_common.hpp"
 
 namespace ptx {
@@ -73,47 +74,97 @@ struct BaseGammaCols {
 };
 
 // ---------------------------------------------------------------- C12: fused forward/loss/adjoint column pass
+// Persistent and software-pipelined: one resident work-group per CU walks the
+// column tiles blockIdx.x, + gridDim.x, ...; while it transforms tile k its
+// registers already hold the loads of tile k + gridDim.x (row-FFT output and
+// measured amplitudes), so the HBM traffic of the next tile overlaps the LDS
+// exchanges of this one.  Buffer accesses keep the addressing to one VGPR per
+// lane.  Per tile: column FFT, A = |Phi|, loss partial,
+// Gamma = gscale (A - Y) Phi / max(A, eps), column iFFT -- the same arithmetic
+// and the same per-tile loss partials as a one-tile-per-work-group launch.
 template <int N, int E, int C>
 __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     k_base_step_cols(float2* __restrict__ tb, const float* __restrict__ measured, const int64_t* __restrict__ order,
                      int64_t count, float* __restrict__ amp_out, float* __restrict__ loss_part, int64_t width,
-                     int64_t tiles_per_frame, float gscale) {
+                     int64_t tiles_per_frame, int64_t ntiles, float gscale) {
   using Cf = ColCfg<N, E>;
   constexpr int T = Cf::T;
   __shared__ float2 lds[C * lds_len<N>()];
   __shared__ float red[16];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
-  const int64_t frame = blockIdx.x / tiles_per_frame;
-  const int64_t tile = blockIdx.x % tiles_per_frame;
-  const int64_t col = tile * C + c;
-  const int64_t off = frame * N * width;  // frame base (uniform)
-  // measured frame of this view: the batch's own (order == null) or measured[order[frame]]
-  int64_t mframe = frame;
-  if (order) {
-    mframe = order[frame];
-    if (mframe < 0 || mframe >= count) mframe = 0;  // flagged by the position gather
+  const int64_t fsz = (int64_t)N * width;               // elements per frame
+  const uint32_t fbytes = (uint32_t)(fsz * 8);
+  const uint32_t lane = (uint32_t)(t * (int)width + c);  // element offset inside a tile (row t, column c)
+  const uint32_t rstep = (uint32_t)(T * width);          // elements from row t + q T to t + (q + 1) T
+  const unsigned nt = (unsigned)ntiles;
+  __shared__ float2 s_tw[N];
+  load_lds_twiddles<N>(s_tw);
+  float2 nv[E];
+  float nm[E];
+  // loads of tile kk into (dv, dm); the measured frame of a view is the
+  // batch's own (order == null) or measured[order[frame]]
+  auto fetch = [&](int64_t kk, float2 (&dv)[E], float (&dm)[E]) {
+    const int64_t blk = xcd_paired_block((unsigned)kk, nt);
+    const int64_t frame = blk / tiles_per_frame;
+    const uint32_t col0 = (uint32_t)((blk % tiles_per_frame) * C);
+    int64_t mframe = frame;
+    if (order) {
+      mframe = order[frame];
+      if (mframe < 0 || mframe >= count) mframe = 0;  // flagged by the position gather
+    }
+    const auto rt = buffer_rsrc(tb + frame * fsz, fbytes);
+    const auto rm = buffer_rsrc(measured + mframe * fsz, fbytes / 2);
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      dv[q] = buf_load_f2(rt, lane * 8u, (q * rstep + col0) * 8u);
+      dm[q] = buf_load_f1(rm, lane * 4u, (q * rstep + col0) * 4u);
+    }
+  };
+  int64_t k = blockIdx.x;
+  if (k < ntiles) fetch(k, nv, nm);
+  for (; k < ntiles; k += gridDim.x) {
+    float2 v[E];
+    float m[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      v[q] = nv[q];
+      m[q] = nm[q];
+    }
+    const int64_t kn = k + gridDim.x;
+    if (kn < ntiles) fetch(kn, nv, nm);  // in flight during this tile's transforms
+    const int64_t blk = xcd_paired_block((unsigned)k, nt);
+    const int64_t frame = blk / tiles_per_frame;
+    const int64_t tile = blk % tiles_per_frame;
+    const uint32_t col0 = (uint32_t)(tile * C);
+    // the exchange addresses depend only on (t, c); recompute them per tile
+    // instead of letting them be hoisted out of the loop and held in registers
+    int tt = t, cc = c;
+    asm volatile("" : "+v"(tt), "+v"(cc));
+    float2* seq_lds = &lds[cc * lds_len<N>()];
+    fft_seq_tw<N, E, -1>(v, tt, seq_lds, s_tw);
+    float lsum = 0.0f;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const float a = c_abs(v[q]);
+      const float d = a - m[q];
+      lsum += d * d;
+      v[q] = c_scale(v[q], gscale * d * fast_rcp(fmaxf(a, kFltEps)));
+      m[q] = a;
+    }
+    if (amp_out) {
+      const auto ra = buffer_rsrc(amp_out + frame * fsz, fbytes / 2);
+#pragma unroll
+      for (int q = 0; q < E; ++q) buf_store_f1(m[q], ra, lane * 4u, (q * rstep + col0) * 4u);
+    }
+    const float tot = block_sum(lsum, red);  // contains the barrier that also orders the LDS reuse below
+    if (threadIdx.x == 0) loss_part[frame * tiles_per_frame + tile] = tot;
+    fft_seq_tw<N, E, +1>(v, tt, seq_lds, s_tw);
+    const auto rt = buffer_rsrc(tb + frame * fsz, fbytes);
+#pragma unroll
+    for (int q = 0; q < E; ++q) buf_store_f2(v[q], rt, lane * 8u, (q * rstep + col0) * 8u);
+    lds_barrier();  // the next tile's first exchange rewrites the LDS
   }
-  const float* meas = measured + mframe * N * width;
-  float2 v[E];
-#pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = tb[off + ((t + q * T) * (int)width + (int)col)];
-  fft_seq<N, E, -1>(v, t, &lds[c * lds_len<N>()]);
-  float lsum = 0.0f;
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int64_t k = off + ((t + q * T) * (int)width + (int)col);
-    const float a = c_abs(v[q]);
-    const float d = a - meas[k - off];
-    lsum += d * d;
-    if (amp_out) amp_out[k] = a;
-    v[q] = c_scale(v[q], gscale * d * fast_rcp(fmaxf(a, kFltEps)));
-  }
-  const float tot = block_sum(lsum, red);  // contains the barrier that also orders the LDS reuse below
-  if (threadIdx.x == 0) loss_part[frame * tiles_per_frame + tile] = tot;
-  fft_seq<N, E, +1>(v, t, &lds[c * lds_len<N>()]);
-#pragma unroll
-  for (int q = 0; q < E; ++q) tb[off + ((t + q * T) * (int)width + (int)col)] = v[q];
 }
 
 // ---------------------------------------------------------------- host launchers
@@ -142,13 +193,15 @@ struct StepColsRun {
     if (width >= Cf::CMAX) {
       constexpr int C = Cf::CMAX;
       const int64_t tiles = width / C;
-      k_base_step_cols<N, E, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(tb, measured, order, count, amp, loss_part,
-                                                                                            width, tiles, gscale);
+      const int64_t grid = persistent_grid(frames * tiles, 1024 / (Cf::T * C));
+      k_base_step_cols<N, E, C><<<dim3((unsigned)grid), dim3(Cf::T * C), 0, s>>>(
+          tb, measured, order, count, amp, loss_part, width, tiles, frames * tiles, gscale);
     } else {
       constexpr int C = 8;
       const int64_t tiles = width / C;
-      k_base_step_cols<N, E, C><<<dim3((unsigned)(frames * tiles)), dim3(Cf::T * C), 0, s>>>(tb, measured, order, count, amp, loss_part,
-                                                                                            width, tiles, gscale);
+      const int64_t grid = persistent_grid(frames * tiles, 1024 / (Cf::T * C));
+      k_base_step_cols<N, E, C><<<dim3((unsigned)grid), dim3(Cf::T * C), 0, s>>>(
+          tb, measured, order, count, amp, loss_part, width, tiles, frames * tiles, gscale);
     }
     return check_launch("base step cols", s);
   }

@@ -309,8 +309,9 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   __shared__ float2 lds[C * lds_len<N>()];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
-  const int64_t tile = blockIdx.x % tiles;
-  const int ch = blockIdx.x / tiles;
+  const int64_t blk = xcd_paired_block(blockIdx.x, gridDim.x);
+  const int64_t tile = blk % tiles;
+  const int ch = (int)(blk / tiles);
   const int64_t col = tile * C + c;
   const int64_t b0 = (int64_t)ch * chunk;
   const int64_t b1 = b0 + chunk < B ? b0 + chunk : B;

@@ -60,6 +60,18 @@ ptx_status check_launch(const char* what, hipStream_t stream) {
   return PTX_OK;
 }
 
+int device_cu_count() {
+  constexpr int kMaxDevices = 64;
+  static std::atomic<int> cache[kMaxDevices];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return 256;
+  int n = cache[dev].load(std::memory_order_relaxed);
+  if (n > 0) return n;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+  cache[dev].store(n, std::memory_order_relaxed);
+  return n;
+}
+
 template <int N, int E>
 struct FftRowsRun {
   static ptx_status run(float2* data, int64_t nrows, int inverse, hipStream_t s) {

@@ -128,8 +128,9 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   __shared__ float red[16];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
-  const int64_t frame = blockIdx.x / tiles_per_frame;
-  const int64_t tile = blockIdx.x % tiles_per_frame;
+  const int64_t blk = xcd_paired_block(blockIdx.x, gridDim.x);
+  const int64_t frame = blk / tiles_per_frame;
+  const int64_t tile = blk % tiles_per_frame;
   const int64_t col = tile * C + c;
   const int64_t off = frame * N * width;  // frame base (uniform)
   float2 v[E];

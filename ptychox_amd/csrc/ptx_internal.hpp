@@ -34,6 +34,19 @@ ptx_status check_launch(const char* what, hipStream_t stream = nullptr);
 ptx_status launch_gather_positions(const int64_t* order, const int64_t* positions_all, int64_t count, int64_t batch,
                                    int64_t* positions_out, int* bad, hipStream_t stream);
 
+// Compute units of the current device (cached per device; 256 on MI355X).
+int device_cu_count();
+
+// Work-groups of a persistent pass over `ntiles` tiles with `per_cu` resident
+// work-groups per compute unit: a multiple of 16 when the grid is at least
+// 16 wide (so the XCD pairing of adjacent tiles carries over to the tile loop).
+inline int64_t persistent_grid(int64_t ntiles, int per_cu) {
+  int64_t g = (int64_t)device_cu_count() * per_cu;
+  if (g > ntiles) g = ntiles;
+  if (g >= 16) g &= ~(int64_t)15;
+  return g < 1 ? 1 : g;
+}
+
 // Records timing event 0 when armed; called by each entry point before its first kernel.
 void timing_begin(hipStream_t stream);
 

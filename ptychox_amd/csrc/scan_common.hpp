@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(256)
         }
       }
     }
-    __syncthreads();  // next view's first LDS write vs this view's last LDS read
+    lds_barrier();  // next view's first LDS write vs this view's last LDS read
   }
   if (rv) {
     float2* d = gp_part + ((int64_t)ch * h + r) * N;
