@@ -58,6 +58,14 @@ typedef enum ptx_status {
 /* Add into the gradient outputs instead of overwriting them. */
 #define PTX_FLAG_ACCUMULATE 2u
 #define PTX_FLAG_ATOMIC_SCATTER 4u
+/* ptx_coded_backward only: the gradient convention of the MATLAB near-field
+ * operator adPtychoX_nearf (matlab/PtychoX_updated/scripts/backward/bwd_adPtychoX_nearf.cu):
+ * the detector step is gZ = Z G_up/(A_up + 0.001) without the 1/ds^2 of the
+ * exact derivative (kernels_nearf.cuh:507-530), and both gradients are returned
+ * conjugated (its mask sum is conj(gY) X, kernels_nearf.cuh:394-436, and its
+ * object gradient is the forward DFT of the conjugated spectrum sum,
+ * kernels_nearf.cuh:186-240).  With transfer_1 = NULL this is that operator. */
+#define PTX_FLAG_MATLAB_NEARF 8u
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* ptx_last_error(void);
@@ -277,6 +285,10 @@ size_t ptx_coded_workspace(int64_t height, int64_t width, int64_t batch, int64_t
  *   sensor_field[b] = ifft2(fft2(x_forward[b] coded_surface) transfer_2);
  *   amplitude[b] = sqrt(max(avgpool_ds |sensor_field[b]|^2, 0))  -> [batch, H/ds, W/ds].
  * shifts_xy: device float [batch][2] normalised shifts (dx, dy).
+ * transfer_1 may be NULL for H1 = 1 (every coded entry point): the full-frame
+ * near-field form of the MATLAB operator, whose first propagator is disabled
+ * (matlab/PtychoX_updated/scripts/forward/fwd_adPtychoX_nearf.cu:63-114,
+ * kernels_nearf.cuh:34-80).
  * Replaces ptychox_coded_forward_cuda (python/ptychoX/csrc/coded_kernel.cu:283-345, bound at
  * coded_extension.cpp:26-56) and the torch path coded/ops.py:136-154.  x_forward and
  * sensor_field are normalised like the torch path (the CUDA extension keeps them
@@ -292,7 +304,9 @@ ptx_status ptx_coded_forward(const void* object_field, int64_t height, int64_t w
  *   grad_coded = sum_b gY conj(X_b); grad_object = N ifft2(sum_b conj(H1 R_b) fft2(gY conj(C))/N).
  * Replaces ptychox_coded_backward_cuda (python/ptychoX/csrc/coded_kernel.cu:347-424, bound at
  * coded_extension.cpp:58-88).  Always deterministic: both shared gradients are
- * accumulated over fixed chunks of views and summed in a fixed order. */
+ * accumulated over fixed chunks of views and summed in a fixed order.
+ * PTX_FLAG_MATLAB_NEARF selects the MATLAB near-field gradient convention
+ * (see the flag); PTX_FLAG_ACCUMULATE adds into the outputs. */
 ptx_status ptx_coded_backward(const float* grad_output, const void* object_spectrum, const void* x_forward,
                               const void* sensor_field, const void* coded_surface, const void* transfer_1,
                               const void* transfer_2, const float* shifts_xy, int64_t batch, int64_t height,

@@ -47,6 +47,9 @@ __all__ = [
     "coded_backward",
     "nearfield_forward",
     "nearfield_backward",
+    "MATLAB_AMP_EPS",
+    "nearfield_matlab_forward",
+    "nearfield_matlab_backward",
     "mse_amplitude_loss",
     "base_train_step",
     "fpm_train_step",
@@ -281,6 +284,75 @@ def coded_backward(
     g_spec = np.sum(np.conj(transfer_1)[None] * np.conj(ramp) * np.fft.fft2(gx), axis=0) / n
     g_object = np.fft.ifft2(g_spec) * n
     return g_object, g_coded
+
+
+# ----------------------------------------------------------------------------
+# MATLAB near-field operator (full-frame form), restated step by step from the
+# MEX sources in their own operation order and normalisations
+# (matlab/PtychoX_updated/@adPtychoX_nearf/adPtychoX_nearf.m:46-91,
+# scripts/forward/fwd_adPtychoX_nearf.cu, scripts/backward/bwd_adPtychoX_nearf.cu,
+# scripts/kernels/kernels_nearf.cuh).  Arrays in logical (row, column) order;
+# x pairs with the column frequency.
+# ----------------------------------------------------------------------------
+
+MATLAB_AMP_EPS = 1.0e-3  # kernels_nearf.cuh:523 (out + 0.001f)
+
+
+def _ifft_unnorm(x: np.ndarray) -> np.ndarray:
+    """cuFFT's CUFFT_INVERSE: the unnormalised inverse DFT (``N * ifft2``)."""
+    n = x.shape[-1] * x.shape[-2]
+    return np.fft.ifft2(x) * n
+
+
+def _signed_freq_phase(h: int, w: int, shifts_xy: np.ndarray) -> np.ndarray:
+    """``2 pi (kx dx + ky dy)`` with signed integer frequencies
+    (``i - N`` for ``2 i >= N``, kernels_nearf.cuh:57-64)."""
+    ky = np.arange(h) - (2 * np.arange(h) >= h) * h
+    kx = np.arange(w) - (2 * np.arange(w) >= w) * w
+    sh = np.asarray(shifts_xy, dtype=np.float64)
+    return 2.0 * np.pi * (sh[:, 0, None, None] * kx[None, None, :] + sh[:, 1, None, None] * ky[None, :, None])
+
+
+def nearfield_matlab_forward(wavefront1, wavefront2, h_d2, shifts_xy, downsample) -> dict:
+    """``observe = adPtychoX_nearf.forward(wavefront1, wavefront2, H_d2, shiftsPos)``
+    (fwd_adPtychoX_nearf.cu:58-114):
+    W1 = fft2(w1); X_b = ifft_unnorm(W1 e^{+i phi_b}) / N (kept as X_record);
+    Z_b = ifft_unnorm(fft2(X_b w2) H2 / N); observe_b = sqrt(mean_{ds x ds} |Z_b|^2)."""
+    w1 = np.asarray(wavefront1, dtype=np.complex128)
+    h, w = w1.shape
+    n = h * w
+    ds = int(downsample)
+    spec = np.fft.fft2(w1)                                                    # cu:72-77
+    x = spec[None] * np.exp(1j * _signed_freq_phase(h, w, shifts_xy))        # cuh:34-80
+    x = _ifft_unnorm(x)                                                       # cu:86-91
+    x_record = x / n                                                          # cuh:256-291 (scaled X written back)
+    z = np.fft.fft2(x_record * np.asarray(wavefront2)[None])                  # cuh:284-288, cu:101-106
+    z = z * np.asarray(h_d2)[None] / n                                        # cuh:306-344
+    z = _ifft_unnorm(z)                                                       # cu:112-117
+    b = z.shape[0]
+    pooled = (np.abs(z) ** 2).reshape(b, h // ds, ds, w // ds, ds).mean(axis=(2, 4))  # cuh:454-488
+    return {"observe": np.sqrt(pooled), "x_record": x_record, "x_forward": z}
+
+
+def nearfield_matlab_backward(dl_dout, fwd: dict, wavefront2, h_d2, shifts_xy, downsample):
+    """``[dldw1, dldw2] = adPtychoX_nearf.backward(...)`` (bwd_adPtychoX_nearf.cu):
+    P_b = Z_b (G_b / (observe_b + 0.001))_up (cuh:507-530, no 1/ds^2);
+    Y_b = ifft_unnorm(fft2(P_b) conj(H2) / N) (cuh:346-392);
+    dldw2 = sum_b conj(Y_b) X_b and F_b = fft2(Y_b conj(w2)) (cuh:394-436);
+    S = sum_b conj(F_b e^{-i phi_b}) / N (cuh:186-240); dldw1 = fft2(S)."""
+    ds = int(downsample)
+    z = fwd["x_forward"]
+    x_record = fwd["x_record"]
+    b, h, w = z.shape
+    n = h * w
+    g = np.asarray(dl_dout, dtype=np.float64) / (fwd["observe"] + MATLAB_AMP_EPS)
+    p = z * np.repeat(np.repeat(g, ds, axis=1), ds, axis=2)
+    y = _ifft_unnorm(np.fft.fft2(p) * np.conj(np.asarray(h_d2))[None] / n)
+    dldw2 = np.sum(np.conj(y) * x_record, axis=0)
+    f = np.fft.fft2(y * np.conj(np.asarray(wavefront2))[None])
+    s = np.sum(np.conj(f * np.exp(-1j * _signed_freq_phase(h, w, shifts_xy))), axis=0) / n
+    dldw1 = np.fft.fft2(s)
+    return dldw1, dldw2
 
 
 # ----------------------------------------------------------------------------

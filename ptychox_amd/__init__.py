@@ -15,6 +15,7 @@ from .fpm import clear_cuda_plan_cache as clear_fpm_cuda_plan_cache
 from .fpm import fourier_ptychography, ptychoX_fourier
 from .nearfield import angular_spectrum_transfer, nearfield_ptychography, ptychoX_nearfield
 from .nearfield import clear_cuda_plan_cache as clear_nearfield_cuda_plan_cache
+from .nearfield_matlab import AdPtychoXNearf, ptychoX_nearfield_matlab
 
 
 def clear_cuda_plan_cache() -> None:
@@ -36,6 +37,8 @@ __all__ = [
     "angular_spectrum_transfer",
     "nearfield_ptychography",
     "ptychoX_nearfield",
+    "AdPtychoXNearf",
+    "ptychoX_nearfield_matlab",
     "fourier_ptychography",
     "ptychoX_fourier",
     "matlab_positions_to_torch",

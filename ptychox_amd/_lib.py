@@ -18,6 +18,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libptychox_hip.so"
 PTX_FLAG_DETERMINISTIC = 1
 PTX_FLAG_ACCUMULATE = 2
 PTX_FLAG_ATOMIC_SCATTER = 4
+PTX_FLAG_MATLAB_NEARF = 8
 PTX_ADAM_PROJECT_REAL_NONNEG = 1
 
 _STATUS = {

@@ -116,7 +116,8 @@ def native_forward(object_field, coded_surface, transfer_1, transfer_2, shifts_x
     return spec, x_fwd, sensor, amp
 
 
-def native_backward(grad_output, spec, x_fwd, sensor, coded_surface, transfer_1, transfer_2, shifts_xy, downsample):
+def native_backward(grad_output, spec, x_fwd, sensor, coded_surface, transfer_1, transfer_2, shifts_xy, downsample,
+                    flags=0):
     H, W = spec.shape
     B = shifts_xy.shape[0]
     ds = int(downsample)
@@ -128,7 +129,7 @@ def native_backward(grad_output, spec, x_fwd, sensor, coded_surface, transfer_1,
         "ptx_coded_backward",
         _lib.ptr(grad_output), _lib.ptr(spec), _lib.ptr(x_fwd), _lib.ptr(sensor), _lib.ptr(coded_surface),
         _lib.ptr(transfer_1), _lib.ptr(transfer_2), _lib.ptr(shifts_xy), B, H, W, ds, _lib.ptr(grad_object),
-        _lib.ptr(grad_coded), _lib.ptr(ws), ws.numel(), 0, _lib.stream_of(dev),
+        _lib.ptr(grad_coded), _lib.ptr(ws), ws.numel(), int(flags), _lib.stream_of(dev),
     )
     return grad_object, grad_coded
 

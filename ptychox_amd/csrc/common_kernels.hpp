@@ -23,16 +23,26 @@ inline void fill_zero2(float2* p, int64_t n, hipStream_t s) {
 }
 
 // out[i] (+)= sum_{c < nparts} part[c*n + i], summed in ascending c (deterministic)
+// conj: out (+)= conj(sum of the parts)
 static __global__ void __launch_bounds__(256) k_sum_parts2(const float2* __restrict__ part, int64_t nparts, int64_t n,
-                                                    float2* __restrict__ out, int accumulate) {
+                                                    float2* __restrict__ out, int accumulate, int conj) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  float2 s = accumulate ? out[i] : c_zero();
-  for (int64_t c = 0; c < nparts; ++c) s = c_add(s, part[c * n + i]);
-  out[i] = s;
+  if (!conj) {
+    float2 s = accumulate ? out[i] : c_zero();
+    for (int64_t c = 0; c < nparts; ++c) s = c_add(s, part[c * n + i]);
+    out[i] = s;
+  } else {
+    float2 t = c_zero();
+    for (int64_t c = 0; c < nparts; ++c) t = c_add(t, part[c * n + i]);
+    t.y = -t.y;
+    out[i] = accumulate ? c_add(out[i], t) : t;
+  }
 }
-inline void sum_parts2(const float2* part, int64_t nparts, int64_t n, float2* out, int accumulate, hipStream_t s) {
-  if (n > 0) k_sum_parts2<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(part, nparts, n, out, accumulate);
+inline void sum_parts2(const float2* part, int64_t nparts, int64_t n, float2* out, int accumulate, hipStream_t s,
+                       int conj = 0) {
+  if (n > 0)
+    k_sum_parts2<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(part, nparts, n, out, accumulate, conj);
 }
 
 // out[b] = scale * sum_{k < nparts} part[b*nparts + k], in ascending k

@@ -29,6 +29,9 @@ namespace ptx {
 namespace {
 
 constexpr float kCodedAmpEps = 1.0e-7f;  // python/ptychoX/csrc/coded_kernel.cu:227
+// The MATLAB near-field backward divides by (A + 0.001) and drops the 1/ds^2
+// of the exact derivative (matlab/PtychoX_updated/scripts/kernels/kernels_nearf.cuh:507-530).
+constexpr float kMatlabAmpEps = 1.0e-3f;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 // ex[b, kx] = exp(i 2 pi dx_b fx),  ey[b, ky] = exp(i 2 pi dy_b fy)   (tables [B, W] and [B, H])
@@ -70,7 +73,8 @@ struct CodedSpecCols {
     for (int q = 0; q < E; ++q) {
       const int64_t r = t + q * T;
       const int64_t k = r * width + col;
-      v[q] = c_mul(c_mul(spec[k], h1[k]), c_mul(ey[frame * H + r], rx));
+      const float2 o = h1 ? c_mul(spec[k], h1[k]) : spec[k];  // h1 == null: H1 = 1
+      v[q] = c_mul(o, c_mul(ey[frame * H + r], rx));
     }
   }
   template <int N, int E>
@@ -142,7 +146,8 @@ struct CodedZRows {
 
 // RMS pooling of |Z|^2 over ds x ds cells.  mode 0: amplitude only;
 // mode 1 (backward): sc = G/(ds^2 max(A, eps)); mode 2 (step): amplitude, loss
-// partial per pooled row and sc = gscale (A - Y)/(ds^2 max(A, eps)).
+// partial per pooled row and sc = gscale (A - Y)/(ds^2 max(A, eps));
+// mode 3 (MATLAB near-field backward): sc = G/(A + 0.001).
 static __global__ void __launch_bounds__(256)
     k_coded_pool(const float2* __restrict__ z, int64_t B, int64_t H, int64_t W, int ds, const float* __restrict__ g,
                  float* __restrict__ amp, float* __restrict__ sc, float* __restrict__ loss_part, float gscale,
@@ -167,6 +172,8 @@ static __global__ void __launch_bounds__(256)
     const float d = a - g[i];
     sc[i] = gscale * d * inv_cell * fast_rcp(fmaxf(a, kCodedAmpEps));
     loss_part[i] = d * d;
+  } else if (mode == 3) {
+    sc[i] = g[i] / (a + kMatlabAmpEps);
   }
 }
 
@@ -178,6 +185,7 @@ struct CodedGradZRows {
   float2* out;
   int64_t H;
   int ds;
+  int matlab;  // ds == 1: gZ = Z G/(|Z| + 0.001) (MATLAB near-field backward)
   template <int N, int E>
   __device__ __forceinline__ void load(int64_t row, bool valid, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
@@ -191,7 +199,8 @@ struct CodedGradZRows {
         v[q] = c_zero();
       } else if (ds == 1) {
         const float2 e = z[row * N + i];
-        v[q] = c_scale(e, srow[i] * fast_rcp(fmaxf(c_abs(e), kCodedAmpEps)));
+        v[q] = matlab ? c_scale(e, srow[i] / (c_abs(e) + kMatlabAmpEps))
+                      : c_scale(e, srow[i] * fast_rcp(fmaxf(c_abs(e), kCodedAmpEps)));
       } else {
         v[q] = c_scale(z[row * N + i], srow[i / ds]);
       }
@@ -318,7 +327,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   float2 hc[E], acc[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) {
-    hc[q] = h1[(t + q * T) * (int)width + (int)col];
+    hc[q] = h1 ? h1[(t + q * T) * (int)width + (int)col] : make_float2(1.0f, 0.0f);  // null: H1 = 1
     acc[q] = c_zero();
   }
   for (int64_t b = b0; b < b1; ++b) {
@@ -570,22 +579,28 @@ ptx_status forward_to_t(const float2* spec, const float2* mask, const float2* t1
   return dispatch_ne<TransferRun>(PASS_COL, H, ws.tb, t2, B, W, false, s);
 }
 
-// Q2..end: T holds row-FFT of gZ; produces grad_object and grad_coded
+// Q2..end: T holds row-FFT of gZ; produces grad_object and grad_coded.
+// matlab: both gradients conjugated, as the MATLAB near-field backward returns
+// them -- grad_coded = conj(sum_b gY conj(X_b)) and
+// grad_object = conj(N ifft2(S)) = fft2(conj(S)) for the spectrum gradient S
+// (matlab/PtychoX_updated/scripts/kernels/kernels_nearf.cuh:186-240,394-436 and
+// scripts/backward/bwd_adPtychoX_nearf.cu, whose last transform is forward).
 ptx_status backward_from_t(const float2* xfwd, const float2* mask, const float2* t1, const float2* t2, int64_t H,
                            int64_t W, int64_t B, const Chunks& c, const CodedWorkspace& ws, float2* gobj,
-                           float2* gcoded, int accumulate, hipStream_t s) {
+                           float2* gcoded, int accumulate, hipStream_t s, bool matlab = false) {
   const float inv_n = 1.0f / (float)(H * W);
   ptx_status st = dispatch_ne<TransferRun>(PASS_COL, H, ws.tb, t2, B, W, true, s);
   if (st != PTX_OK) return st;
   if ((st = dispatch_ne<YRowsRun>(PASS_ROW_LOOP, W, ws.tb, xfwd, mask, H, B, c, inv_n, ws.gc_part, s)) != PTX_OK) return st;
   if ((st = dispatch_ne<SpecGradColsRun>(PASS_COL_LOOP, H, ws.tb, t1, ws.ex, ws.ey, B, W, c, inv_n, ws.gs_part, s)) != PTX_OK)
     return st;
-  sum_parts2(ws.gc_part, c.row_nchunks, H * W, gcoded, accumulate, s);
+  sum_parts2(ws.gc_part, c.row_nchunks, H * W, gcoded, accumulate, s, matlab ? 1 : 0);
   if ((st = check_launch("mask partial sum", s)) != PTX_OK) return st;
-  sum_parts2(ws.gs_part, c.col_nchunks, H * W, ws.gspec, 0, s);
+  sum_parts2(ws.gs_part, c.col_nchunks, H * W, ws.gspec, 0, s, matlab ? 1 : 0);
   if ((st = check_launch("spectrum partial sum", s)) != PTX_OK) return st;
-  // grad_O = N ifft2(grad_O^) = unnormalised inverse DFT; rows in place, columns into the output
-  return fft2_frame(ws.gspec, gobj, H, W, +1, accumulate, ws.gspec, s);
+  // grad_O = N ifft2(grad_O^) = unnormalised inverse DFT (MATLAB: the forward
+  // DFT of the conjugate); rows in place, columns into the output
+  return fft2_frame(ws.gspec, gobj, H, W, matlab ? -1 : +1, accumulate, ws.gspec, s);
 }
 
 }  // namespace
@@ -601,7 +616,7 @@ ptx_status ptx_coded_forward(const void* object_field, int64_t H, int64_t W, con
                              float* amplitude, void* workspace, size_t ws_bytes, void* stream) {
   ptx_status st = check("ptx_coded_forward", H, W, B, downsample);
   if (st != PTX_OK) return st;
-  if (!object_field || !object_spectrum || (B > 0 && (!coded_surface || !transfer_1 || !transfer_2 || !shifts_xy ||
+  if (!object_field || !object_spectrum || (B > 0 && (!coded_surface || !transfer_2 || !shifts_xy ||
                                                       !x_forward || !sensor_field || !amplitude))) {
     set_error("ptx_coded_forward: null pointer argument");
     return PTX_ERR_INVALID_ARGUMENT;
@@ -658,8 +673,8 @@ ptx_status ptx_coded_backward(const float* grad_output, const void* object_spect
     }
     return check_launch("zero fill", s);
   }
-  if (!grad_output || !object_spectrum || !x_forward || !sensor_field || !coded_surface || !transfer_1 ||
-      !transfer_2 || !shifts_xy) {
+  if (!grad_output || !object_spectrum || !x_forward || !sensor_field || !coded_surface || !transfer_2 ||
+      !shifts_xy) {
     set_error("ptx_coded_backward: null pointer argument");
     return PTX_ERR_INVALID_ARGUMENT;
   }
@@ -669,20 +684,24 @@ ptx_status ptx_coded_backward(const float* grad_output, const void* object_spect
     set_error("ptx_coded_backward: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
     return PTX_ERR_WORKSPACE;
   }
+  const bool matlab = (flags & PTX_FLAG_MATLAB_NEARF) != 0;
   if ((st = ramps(shifts_xy, B, H, W, ws, s)) != PTX_OK) return st;
   const float* scmap = grad_output;
   if (downsample > 1) {
-    // scale map G/(ds^2 max(A, eps)) from the recomputed amplitude (coded_kernel.cu:370-377)
+    // scale map G/(ds^2 max(A, eps)) from the recomputed amplitude (coded_kernel.cu:370-377),
+    // or G/(A + 0.001) for the MATLAB near-field backward
     const int64_t n = B * (H / downsample) * (W / downsample);
     k_coded_pool<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(
-        (const float2*)sensor_field, B, H, W, (int)downsample, grad_output, nullptr, ws.sc, nullptr, 0.0f, 1);
+        (const float2*)sensor_field, B, H, W, (int)downsample, grad_output, nullptr, ws.sc, nullptr, 0.0f,
+        matlab ? 3 : 1);
     if ((st = check_launch("coded pool", s)) != PTX_OK) return st;
     scmap = ws.sc;
   }
-  CodedGradZRows q1{(const float2*)sensor_field, scmap, ws.tb, H, (int)downsample};
+  CodedGradZRows q1{(const float2*)sensor_field, scmap, ws.tb, H, (int)downsample, matlab ? 1 : 0};
   if ((st = dispatch_ne<GradZRowsRun>(PASS_ROW, W, q1, B * H, s)) != PTX_OK) return st;
   return backward_from_t((const float2*)x_forward, (const float2*)coded_surface, (const float2*)transfer_1,
-                         (const float2*)transfer_2, H, W, B, c, ws, (float2*)grad_object, (float2*)grad_coded, acc, s);
+                         (const float2*)transfer_2, H, W, B, c, ws, (float2*)grad_object, (float2*)grad_coded, acc, s,
+                         matlab);
 }
 
 ptx_status ptx_coded_step(const void* object_field, int64_t H, int64_t W, const void* coded_surface,
@@ -706,7 +725,7 @@ ptx_status ptx_coded_step(const void* object_field, int64_t H, int64_t W, const 
     }
     return check_launch("zero fill", s);
   }
-  if (!object_field || !coded_surface || !transfer_1 || !transfer_2 || !shifts_xy || !measured ||
+  if (!object_field || !coded_surface || !transfer_2 || !shifts_xy || !measured ||
       !loss_per_position) {
     set_error("ptx_coded_step: null pointer argument");
     return PTX_ERR_INVALID_ARGUMENT;
@@ -741,7 +760,7 @@ ptx_status ptx_coded_step(const void* object_field, int64_t H, int64_t W, const 
     if ((st = check_launch("coded pool", s)) != PTX_OK) return st;
     loss_rows(ws.loss_part, lh * lw, B, 1.0f / (float)(lh * lw), loss_per_position, s);
     if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
-    CodedGradZRows q1{ws.tb, ws.sc, ws.tb, H, (int)ds};
+    CodedGradZRows q1{ws.tb, ws.sc, ws.tb, H, (int)ds, 0};
     if ((st = dispatch_ne<GradZRowsRun>(PASS_ROW, W, q1, B * H, s)) != PTX_OK) return st;
   }
   return backward_from_t(ws.xfwd, (const float2*)coded_surface, (const float2*)transfer_1, (const float2*)transfer_2,

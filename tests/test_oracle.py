@@ -370,3 +370,65 @@ def test_torch_literal_path_matches_oracle():
     assert abs(float(l32) - loss) / loss < 1e-5
     assert nrel(gs32.numpy(), gs) < 1e-5
     assert nrel(gp32.numpy(), gp) < 1e-5
+
+
+# ---------------------------------------------------------------- MATLAB near-field operator
+
+def test_nearfield_matlab_forward_closed_form_and_coded_form():
+    rng = np.random.default_rng(21)
+    n = 16
+    w1 = crandn(n, n, rng=rng)
+    # unit diffuser and propagator, integer shifts t: observe = |roll(w1, -t)|
+    t = np.array([[3, -2], [0, 5], [-7, 1]])
+    obs = po.nearfield_matlab_forward(w1, np.ones((n, n)), np.ones((n, n)), t / n, 1)["observe"]
+    for b in range(3):
+        expect = np.abs(np.roll(w1, shift=(-t[b, 1], -t[b, 0]), axis=(0, 1)))
+        assert np.allclose(obs[b], expect, atol=1e-12)
+    # general case: the coded forward with transfer_1 = 1 (fwd_adPtychoX_nearf.cu:63-114)
+    w2, h2 = crandn(n, n, rng=rng), np.exp(1j * rng.uniform(0, 2 * np.pi, (n, n)))
+    sh = rng.uniform(-0.3, 0.3, (4, 2))
+    for ds in (1, 2, 4):
+        a = po.nearfield_matlab_forward(w1, w2, h2, sh, ds)["observe"]
+        b = po.coded_forward(w1, w2, np.ones((n, n)), h2, sh, ds)["amplitude"]
+        assert nrel(a, b) < 1e-13
+
+
+@pytest.mark.parametrize("ds", [1, 2])
+def test_nearfield_matlab_backward_is_conjugated_adjoint(ds):
+    # the MATLAB backward equals the conjugate of the (finite-difference pinned)
+    # coded adjoint with H1 = 1, once its detector step Z G/(A + 0.001) is
+    # written as the exact one's Z G'/(ds^2 max(A, eps))
+    rng = np.random.default_rng(22 + ds)
+    n = 16
+    w1, w2 = crandn(n, n, rng=rng), crandn(n, n, rng=rng)
+    h2 = np.exp(1j * rng.uniform(0, 2 * np.pi, (n, n)))
+    sh = rng.uniform(-0.3, 0.3, (3, 2))
+    f = po.nearfield_matlab_forward(w1, w2, h2, sh, ds)
+    g = rng.standard_normal(f["observe"].shape)
+    d1, d2 = po.nearfield_matlab_backward(g, f, w2, h2, sh, ds)
+    c = po.coded_forward(w1, w2, np.ones((n, n)), h2, sh, ds)
+    a = c["amplitude"]
+    g_exact = g * ds * ds * np.maximum(a, po.CODED_AMP_EPS) / (a + po.MATLAB_AMP_EPS)
+    e1, e2 = po.coded_backward(g_exact, c, w2, np.ones((n, n)), h2, ds)
+    assert nrel(d1, np.conj(e1)) < 1e-13
+    assert nrel(d2, np.conj(e2)) < 1e-13
+    # and it is not the exact gradient: the detector step and conjugation differ
+    x1, _ = po.coded_backward(g, c, w2, np.ones((n, n)), h2, ds)
+    assert nrel(d1, x1) > 1e-3
+
+
+def test_nearfield_matlab_column_major_layout():
+    # column-major memory viewed row-major is the per-frame transpose: the same
+    # operator with dx and dy swapped returns the transposed frames
+    rng = np.random.default_rng(25)
+    n = 16
+    w1, w2 = crandn(n, n, rng=rng), crandn(n, n, rng=rng)
+    h2 = np.exp(1j * rng.uniform(0, 2 * np.pi, (n, n)))
+    sh = rng.uniform(-0.3, 0.3, (3, 2))
+    f = po.nearfield_matlab_forward(w1, w2, h2, sh, 2)
+    ft = po.nearfield_matlab_forward(w1.T, w2.T, h2.T, sh[:, ::-1], 2)
+    assert nrel(ft["observe"], np.swapaxes(f["observe"], 1, 2)) < 1e-13
+    g = rng.standard_normal(f["observe"].shape)
+    d1, d2 = po.nearfield_matlab_backward(g, f, w2, h2, sh, 2)
+    t1, t2 = po.nearfield_matlab_backward(np.swapaxes(g, 1, 2), ft, w2.T, h2.T, sh[:, ::-1], 2)
+    assert nrel(t1, d1.T) < 1e-13 and nrel(t2, d2.T) < 1e-13
