@@ -321,6 +321,46 @@ ptx_status ptx_coded_step(const void* object_field, int64_t height, int64_t widt
                           float* amplitude, void* grad_object, void* grad_coded, void* workspace,
                           size_t workspace_bytes, unsigned flags, void* stream);
 
+/* ------------------------------------------------------------------ alternative losses
+ * Stencil losses on stacks of frames [batch][h][w], with their gradients.
+ * With d = pred - target (or the field), forward differences with wrap-around
+ * dx(i,j) = d(i,j+1) - d(i,j), dy(i,j) = d(i+1,j) - d(i,j) (torch.roll(d, -1),
+ * MATLAB d([2:end,1])) and a per-site penalty phi:
+ *   PTX_LOSS_L2           phi = |d|^2              (+loss_fun/l2_loss.m; mse_loss "mean" with scale 1/numel)
+ *   PTX_LOSS_FD_ISO       phi = sqrt(|dx|^2 + |dy|^2 + eps)
+ *                         (python/train_coded.py:101-106, eps 1e-5; +loss_fun/tv_loss.m 'isotropic' with eps 0)
+ *   PTX_LOSS_FD_ANISO     phi = |dx| + |dy|        (python/train_coded.py:107; +loss_fun/fd_loss.m, tv_loss.m)
+ *   PTX_LOSS_GRAD_L2      phi = |dx|^2 + |dy|^2    (+loss_fun/tv_loss.m, sparse_loss.m 'L2')
+ *   PTX_LOSS_SPARSE_ISO   phi = 1 - exp(-8 sqrt(|dx|^2 + |dy|^2 + eps))   (+loss_fun/sparse_loss.m, eps 1e-5)
+ *   PTX_LOSS_SPARSE_ANISO phi = 1 - exp(-8 (|dx| + |dy|))                 (+loss_fun/sparse_loss.m)
+ * loss_per_frame[b] = scale * sum_ij phi; grad = scale * dL/dpred in PyTorch's
+ * convention (real: dL/dx; complex: dL/dRe + i dL/dIm).  |.| has the zero
+ * subgradient at 0 (torch.abs backward); with eps = 0 a flat site contributes
+ * a zero gradient.  Sums are deterministic (fixed-order work-group and frame
+ * reductions).  workspace: ptx_loss_workspace bytes. */
+#define PTX_LOSS_L2 0
+#define PTX_LOSS_FD_ISO 1
+#define PTX_LOSS_FD_ANISO 2
+#define PTX_LOSS_GRAD_L2 3
+#define PTX_LOSS_SPARSE_ISO 4
+#define PTX_LOSS_SPARSE_ANISO 5
+
+size_t ptx_loss_workspace(int64_t batch, int64_t h, int64_t w);
+
+/* Residual loss of predicted amplitudes (pred, target: device float [batch][h][w];
+ * target may be NULL for d = pred).  The loss step of the reference training
+ * drivers (python/train_coded.py:234-237 with --loss fd-isotropic / fd-anisotropic),
+ * the upstream gradient of ptx_*_backward. */
+ptx_status ptx_amplitude_loss(int kind, const float* pred, const float* target, int64_t batch, int64_t h, int64_t w,
+                              float scale, float eps, float* loss_per_frame, float* grad, void* workspace,
+                              size_t workspace_bytes, void* stream);
+
+/* Regulariser of complex fields (device complex64 [batch][h][w]), the MATLAB
+ * examples' tv_loss / sparse_loss of an object (matlab/Examples/Ptychography/+loss_fun). */
+ptx_status ptx_field_regularizer(int kind, const void* field, int64_t batch, int64_t h, int64_t w, float scale,
+                                 float eps, float* loss_per_frame, void* grad, void* workspace,
+                                 size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,9 @@ __all__ = [
     "MATLAB_AMP_EPS",
     "nearfield_matlab_forward",
     "nearfield_matlab_backward",
+    "LOSS_KINDS",
+    "stencil_loss",
+    "hessian_loss",
     "mse_amplitude_loss",
     "base_train_step",
     "fpm_train_step",
@@ -394,6 +397,72 @@ def nearfield_backward(grad_output, sample_shape, probe, transfer, patches, fiel
 # Amplitude loss of the training drivers (python/train_ptychography.py:555,
 # python/train_fpm.py:261, python/train_coded.py:235: F.mse_loss, mean)
 # ----------------------------------------------------------------------------
+
+# ----------------------------------------------------------------------------
+# Alternative losses (python/train_coded.py:101-107; matlab/Examples/
+# Ptychography/+loss_fun/{l2,fd,tv,sparse,hessian}_loss.m)
+# ----------------------------------------------------------------------------
+
+LOSS_KINDS = ("l2", "fd_iso", "fd_aniso", "grad_l2", "sparse_iso", "sparse_aniso")
+
+
+def _sgn(z):
+    m = np.abs(z)
+    return np.where(m > 0, z / np.where(m > 0, m, 1.0), 0.0)
+
+
+def stencil_loss(kind: str, d: np.ndarray, eps: float = 0.0):
+    """Per-frame sums of the penalty of ``d`` (real or complex, [..., h, w])
+    and its gradient (PyTorch convention), with forward differences
+    ``dx = roll(d, -1, -1) - d``, ``dy = roll(d, -1, -2) - d``
+    (train_coded.py:102-104, +loss_fun/tv_loss.m:2-3)."""
+    d = np.asarray(d)
+    if kind == "l2":
+        return np.sum(np.abs(d) ** 2, axis=(-2, -1)), 2.0 * d
+    dx = np.roll(d, -1, axis=-1) - d
+    dy = np.roll(d, -1, axis=-2) - d
+    mx, my = np.abs(dx), np.abs(dy)
+    if kind in ("fd_iso", "sparse_iso"):
+        s = np.sqrt(mx ** 2 + my ** 2 + eps)
+        k = np.where(s > 0, 1.0 / np.where(s > 0, s, 1.0), 0.0)
+        phi = s
+        if kind == "sparse_iso":
+            e = np.exp(-8.0 * s)
+            phi, k = 1.0 - e, 8.0 * e * k
+        ax, ay = dx * k, dy * k
+    elif kind in ("fd_aniso", "sparse_aniso"):
+        phi = mx + my
+        ax, ay = _sgn(dx), _sgn(dy)
+        if kind == "sparse_aniso":
+            e = np.exp(-8.0 * phi)
+            phi, ax, ay = 1.0 - e, 8.0 * e * ax, 8.0 * e * ay
+    elif kind == "grad_l2":
+        phi = mx ** 2 + my ** 2
+        ax, ay = 2.0 * dx, 2.0 * dy
+    else:
+        raise ValueError(kind)
+    grad = np.roll(ax, 1, axis=-1) + np.roll(ay, 1, axis=-2) - ax - ay
+    return np.sum(phi, axis=(-2, -1)), grad
+
+
+def hessian_loss(dw: np.ndarray, kind: str) -> float:
+    """+loss_fun/hessian_loss.m: second differences dwdxx, dwdyy, dwdxy of the
+    wrapped first differences; 'isotropic' sqrt(|dxx|^2 + |dyy|^2 + 2|dxy|^2),
+    'anisotropic' |dxx| + |dyy| + 2|dxy|, 'L2' |dx|^2 + |dy|^2; summed."""
+    dw = np.asarray(dw)
+    dx = np.roll(dw, -1, axis=-1) - dw   # dW(:,[2:end,1]) - dW
+    dy = np.roll(dw, -1, axis=-2) - dw   # dW([2:end,1],:) - dW
+    if kind == "L2":
+        return float(np.sum(np.abs(dx) ** 2 + np.abs(dy) ** 2))
+    dxx = np.roll(dx, -1, axis=-1) - dx
+    dyy = np.roll(dy, -1, axis=-2) - dy
+    dxy = np.roll(dx, -1, axis=-2) - dx
+    if kind == "isotropic":
+        return float(np.sum(np.sqrt(np.abs(dxx) ** 2 + np.abs(dyy) ** 2 + 2 * np.abs(dxy) ** 2)))
+    if kind == "anisotropic":
+        return float(np.sum(np.abs(dxx) + np.abs(dyy) + 2 * np.abs(dxy)))
+    raise ValueError(kind)
+
 
 def mse_amplitude_loss(amplitude: np.ndarray, measured: np.ndarray):
     """Returns ``(L, L_b, G)`` with ``L = mean (A - Y)^2``, per-position

@@ -19,6 +19,13 @@ PTX_FLAG_DETERMINISTIC = 1
 PTX_FLAG_ACCUMULATE = 2
 PTX_FLAG_ATOMIC_SCATTER = 4
 PTX_FLAG_MATLAB_NEARF = 8
+
+PTX_LOSS_L2 = 0
+PTX_LOSS_FD_ISO = 1
+PTX_LOSS_FD_ANISO = 2
+PTX_LOSS_GRAD_L2 = 3
+PTX_LOSS_SPARSE_ISO = 4
+PTX_LOSS_SPARSE_ANISO = 5
 PTX_ADAM_PROJECT_REAL_NONNEG = 1
 
 _STATUS = {
@@ -65,6 +72,9 @@ SIGNATURES: dict[str, tuple] = {
         [_p, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _p, _i64, _f, _p, _p, _p, _p, _p, _sz, _u, _p],
     ),
     "ptx_adam_step": (_st, [_p, _p, _p, _p, _i64, _f, _f, _p, _u, _p]),
+    "ptx_loss_workspace": (_sz, [_i64, _i64, _i64]),
+    "ptx_amplitude_loss": (_st, [_i, _p, _p, _i64, _i64, _i64, _f, _f, _p, _p, _p, _sz, _p]),
+    "ptx_field_regularizer": (_st, [_i, _p, _i64, _i64, _i64, _f, _f, _p, _p, _p, _sz, _p]),
     "ptx_gather_positions": (_st, [_p, _p, _i64, _i64, _p, _p, _p]),
     "ptx_fpm_forward": (_st, [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _p, _p, _p]),
     "ptx_fpm_backward_workspace": (_sz, [_i64, _i64, _i64, _i64, _i64, _u]),

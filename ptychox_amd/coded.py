@@ -163,28 +163,47 @@ class PtychoXCodedFunction(torch.autograd.Function):
 
 
 class CodedStep:
-    """Fused coded training step on the HIP path (``ptx_coded_step``)."""
+    """Fused coded training step on the HIP path (``ptx_coded_step``).
 
-    def __init__(self, shape, batch, downsample, device, keep_amplitude=False):
+    ``loss`` is the coded driver's ``--loss`` (python/train_coded.py:234-237):
+    ``"mse"`` (``F.mse_loss``, fused into the step, ``loss_scale`` defaults to
+    ``1/(B h w)``) or ``"fd-isotropic"`` / ``"fd-anisotropic"`` (``fd_loss``,
+    a sum, ``loss_scale`` defaults to 1), which run the native forward, the
+    stencil loss writing the upstream gradient, and the native adjoint."""
+
+    LOSSES = ("mse", "fd-isotropic", "fd-anisotropic")
+
+    def __init__(self, shape, batch, downsample, device, keep_amplitude=False, loss="mse"):
         self.H, self.W = (int(s) for s in shape)
         require_native_frame(self.H, self.W, "object_field")
         self.batch = int(batch)
         self.ds = int(downsample)
+        if loss not in self.LOSSES:
+            raise ValueError(f"loss must be one of {self.LOSSES}")
+        self.loss_name = loss
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
         self.ws = _lib.workspace(self._ws_bytes(), self.device)
         self.loss = torch.empty(self.batch, dtype=torch.float32, device=self.device)
+        h, w = self.H // self.ds, self.W // self.ds
         self.amplitude = (
-            torch.empty((self.batch, self.H // self.ds, self.W // self.ds), dtype=torch.float32, device=self.device)
-            if keep_amplitude
+            torch.empty((self.batch, h, w), dtype=torch.float32, device=self.device)
+            if keep_amplitude or loss != "mse"
             else None
         )
+        if loss != "mse":
+            dev, B = self.device, self.batch
+            self._spec = torch.empty((self.H, self.W), dtype=torch.complex64, device=dev)
+            self._x_fwd = torch.empty((B, self.H, self.W), dtype=torch.complex64, device=dev)
+            self._sensor = torch.empty_like(self._x_fwd)
+            self._g = torch.empty((B, h, w), dtype=torch.float32, device=dev)
+            self._loss_ws = _lib.workspace(_lib.query("ptx_loss_workspace", B, h, w), dev)
 
     def _ws_bytes(self) -> int:
         # the size depends on the FFT schedule in the tuning table, so it is
         # re-queried (host-side, no GPU work) before every call
-        return _lib.query("ptx_coded_workspace", self.H, self.W, self.batch, self.ds, 1)
+        return _lib.query("ptx_coded_workspace", self.H, self.W, self.batch, self.ds, 1 if self.loss_name == "mse" else 0)
 
     def __call__(self, object_field, coded_surface, transfer_1, transfer_2, shifts_xy, measured, grad_object,
                  grad_coded, loss_scale=None, batch=None, accumulate=False):
@@ -193,22 +212,49 @@ class CodedStep:
             raise ValueError(f"batch {B} exceeds the planned batch {self.batch}")
         for name, t, dt in (("object_field", object_field, torch.complex64),
                             ("coded_surface", coded_surface, torch.complex64),
-                            ("transfer_1", transfer_1, torch.complex64), ("transfer_2", transfer_2, torch.complex64),
+                            ("transfer_2", transfer_2, torch.complex64),
                             ("shifts_xy", shifts_xy, torch.float32), ("measured", measured, torch.float32),
                             ("grad_object", grad_object, torch.complex64),
                             ("grad_coded", grad_coded, torch.complex64)):
             if t.device != self.device or t.dtype != dt or not t.is_contiguous():
                 raise ValueError(f"{name} must be a contiguous {dt} tensor on {self.device}")
-        if loss_scale is None:
-            loss_scale = 1.0 / (B * (self.H // self.ds) * (self.W // self.ds))
+        if transfer_1 is not None and (transfer_1.device != self.device or transfer_1.dtype != torch.complex64
+                                       or not transfer_1.is_contiguous()):
+            raise ValueError(f"transfer_1 must be None or a contiguous complex64 tensor on {self.device}")
+        h, w = self.H // self.ds, self.W // self.ds
         flags = _lib.PTX_FLAG_ACCUMULATE if accumulate else 0
         if self._ws_bytes() > self.ws.numel():
             self.ws = _lib.workspace(self._ws_bytes(), self.device)
+        stream = _lib.stream_of(self.device)
+        if self.loss_name == "mse":
+            if loss_scale is None:
+                loss_scale = 1.0 / (B * h * w)
+            _lib.call(
+                "ptx_coded_step",
+                _lib.ptr(object_field), self.H, self.W, _lib.ptr(coded_surface), _lib.ptr(transfer_1),
+                _lib.ptr(transfer_2), _lib.ptr(shifts_xy), B, self.ds, _lib.ptr(measured), float(loss_scale),
+                _lib.ptr(self.loss), _lib.ptr(self.amplitude), _lib.ptr(grad_object), _lib.ptr(grad_coded),
+                _lib.ptr(self.ws), self.ws.numel(), flags, stream,
+            )
+            return self.loss[:B]
+        from .losses import FD_EPS, native_amplitude_loss
+
+        if loss_scale is None:
+            loss_scale = 1.0
+        iso = self.loss_name == "fd-isotropic"
         _lib.call(
-            "ptx_coded_step",
+            "ptx_coded_forward",
             _lib.ptr(object_field), self.H, self.W, _lib.ptr(coded_surface), _lib.ptr(transfer_1),
-            _lib.ptr(transfer_2), _lib.ptr(shifts_xy), B, self.ds, _lib.ptr(measured), float(loss_scale),
-            _lib.ptr(self.loss), _lib.ptr(self.amplitude), _lib.ptr(grad_object), _lib.ptr(grad_coded),
-            _lib.ptr(self.ws), self.ws.numel(), flags, _lib.stream_of(self.device),
+            _lib.ptr(transfer_2), _lib.ptr(shifts_xy), B, self.ds, _lib.ptr(self._spec), _lib.ptr(self._x_fwd),
+            _lib.ptr(self._sensor), _lib.ptr(self.amplitude), _lib.ptr(self.ws), self.ws.numel(), stream,
+        )
+        native_amplitude_loss("fd_iso" if iso else "fd_aniso", self.amplitude[:B], measured[:B], loss_scale,
+                              FD_EPS if iso else 0.0, self.loss[:B], self._g[:B], self._loss_ws)
+        _lib.call(
+            "ptx_coded_backward",
+            _lib.ptr(self._g), _lib.ptr(self._spec), _lib.ptr(self._x_fwd), _lib.ptr(self._sensor),
+            _lib.ptr(coded_surface), _lib.ptr(transfer_1), _lib.ptr(transfer_2), _lib.ptr(shifts_xy), B, self.H,
+            self.W, self.ds, _lib.ptr(grad_object), _lib.ptr(grad_coded), _lib.ptr(self.ws), self.ws.numel(), flags,
+            stream,
         )
         return self.loss[:B]

@@ -432,3 +432,83 @@ def test_nearfield_matlab_column_major_layout():
     d1, d2 = po.nearfield_matlab_backward(g, f, w2, h2, sh, 2)
     t1, t2 = po.nearfield_matlab_backward(np.swapaxes(g, 1, 2), ft, w2.T, h2.T, sh[:, ::-1], 2)
     assert nrel(t1, d1.T) < 1e-13 and nrel(t2, d2.T) < 1e-13
+
+
+# ---------------------------------------------------------------- alternative losses
+
+def _loop_penalty(kind, d, eps):
+    # explicit per-site loops with modular neighbours (no roll)
+    B, h, w = d.shape
+    out = np.zeros(B)
+    for b in range(B):
+        for i in range(h):
+            for j in range(w):
+                dx = d[b, i, (j + 1) % w] - d[b, i, j]
+                dy = d[b, (i + 1) % h, j] - d[b, i, j]
+                if kind == "l2":
+                    v = abs(d[b, i, j]) ** 2
+                elif kind == "fd_iso":
+                    v = np.sqrt(abs(dx) ** 2 + abs(dy) ** 2 + eps)
+                elif kind == "fd_aniso":
+                    v = abs(dx) + abs(dy)
+                elif kind == "grad_l2":
+                    v = abs(dx) ** 2 + abs(dy) ** 2
+                elif kind == "sparse_iso":
+                    v = 1 - np.exp(-8 * np.sqrt(abs(dx) ** 2 + abs(dy) ** 2 + eps))
+                else:
+                    v = 1 - np.exp(-8 * (abs(dx) + abs(dy)))
+                out[b] += v
+    return out
+
+
+@pytest.mark.parametrize("kind", po.LOSS_KINDS)
+@pytest.mark.parametrize("cplx", [False, True])
+def test_stencil_loss_values_and_fd_gradient(kind, cplx):
+    rng = np.random.default_rng(31 + po.LOSS_KINDS.index(kind) + 7 * cplx)
+    d = rng.standard_normal((2, 5, 6)) * 0.3
+    if cplx:
+        d = d + 1j * rng.standard_normal((2, 5, 6)) * 0.3
+    eps = 1e-5 if kind in ("fd_iso", "sparse_iso") else 0.0
+    L, g = po.stencil_loss(kind, d, eps)
+    assert np.allclose(L, _loop_penalty(kind, d, eps), rtol=1e-12, atol=1e-12)
+    f = lambda z: float(np.sum(po.stencil_loss(kind, z, eps)[0]))
+    # real data are checked as complex points: the imaginary partials are 0 there
+    # (a small step: the square-root penalties curve sharply where |dx|, |dy| are small)
+    _fd_check(f, d.astype(np.complex128), g.astype(np.complex128), _pick(d.shape, 14, rng), h=1e-6, rtol=1e-6)
+
+
+def test_fd_loss_matches_torch_autograd_of_the_reference_formula():
+    # train_coded.py:101-107 evaluated with torch autograd, FP64
+    rng = np.random.default_rng(35)
+    pred, target = rng.random((3, 8, 10)), rng.random((3, 8, 10))
+    for iso, kind in ((True, "fd_iso"), (False, "fd_aniso")):
+        p = torch.tensor(pred, requires_grad=True)
+        diff = p - torch.tensor(target)
+        dx = torch.roll(diff, shifts=-1, dims=-1) - diff
+        dy = torch.roll(diff, shifts=-1, dims=-2) - diff
+        loss = torch.sqrt(dx.square() + dy.square() + 1e-5).sum() if iso else (dx.abs() + dy.abs()).sum()
+        loss.backward()
+        L, g = po.stencil_loss(kind, pred - target, 1e-5 if iso else 0.0)
+        assert abs(L.sum() - loss.item()) < 1e-10 * abs(loss.item())
+        assert np.allclose(g, p.grad.numpy(), rtol=1e-10, atol=1e-12)
+
+
+def test_hessian_loss_loops():
+    rng = np.random.default_rng(36)
+    w = rng.standard_normal((4, 5)) + 1j * rng.standard_normal((4, 5))
+    h_, w_ = w.shape
+    x = lambda i, j: w[i % h_, j % w_]
+    iso = an = l2 = 0.0
+    for i in range(h_):
+        for j in range(w_):
+            dx = lambda a, b: x(a, b + 1) - x(a, b)
+            dy = lambda a, b: x(a + 1, b) - x(a, b)
+            dxx = dx(i, j + 1) - dx(i, j)
+            dyy = dy(i + 1, j) - dy(i, j)
+            dxy = dx(i + 1, j) - dx(i, j)
+            iso += np.sqrt(abs(dxx) ** 2 + abs(dyy) ** 2 + 2 * abs(dxy) ** 2)
+            an += abs(dxx) + abs(dyy) + 2 * abs(dxy)
+            l2 += abs(dx(i, j)) ** 2 + abs(dy(i, j)) ** 2
+    assert np.isclose(po.hessian_loss(w, "isotropic"), iso)
+    assert np.isclose(po.hessian_loss(w, "anisotropic"), an)
+    assert np.isclose(po.hessian_loss(w, "L2"), l2)
