@@ -33,6 +33,12 @@ HBM_PEAK_GBS = 8000.0
 
 OBJ, PROBE, BATCH = 2048, 512, 64
 
+# HBM bytes moved per launch, from rocprofv3 byte counters on this workload
+# (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a plain 256 MiB copy kernel
+# reads back; profiles/base_step_obj2048_probe512_b64_pmc_traffic.md).  The
+# counters need their own profiled runs, so the figure is recorded here.
+MEASURED_TRAFFIC = {"base step cols": 336_100_000}
+
 
 def parse_args(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -503,7 +509,8 @@ def main(argv=None) -> int:
                    "object": OBJ, "probe": PROBE, "positions": P, "mini_batch": BATCH,
                    "object_gradient": "atomic" if args.atomic else "sorted-segment"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get(dom), "algorithmic_bytes": int(dom_bytes)},
         "path_roofline": {"bytes_per_pattern": b_alg, "achieved": round(path_gbs, 1), "unit": "GB/s",
                           "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
