@@ -39,6 +39,10 @@ OBJ, PROBE, BATCH = 2048, 512, 64
 # counters need their own profiled runs, so the figure is recorded here.
 MEASURED_TRAFFIC = {"base step cols": 336_100_000}
 
+# Stated latency bound of the served mix: p99 from arrival to completion at
+# every measured Poisson rate up to 0.9x the server's saturation throughput.
+P99_BOUND_MS = 100.0
+
 
 def parse_args(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -283,7 +287,8 @@ def measure_serving(dev, args) -> dict:
 
     sv.Server(dev).run(base[:10], probs)  # warm-up: first launches, graph machinery
     srv = sv.Server(dev)
-    sat = sv.summarize(srv.run(base, probs))
+    sat_done = srv.run(base, probs)
+    sat = sv.summarize(sat_done)
     sat["peak_in_flight"] = srv.peak_in_flight
     sat["peak_resident_gb"] = round(srv.peak_bytes / 1e9, 3)
     print(f"[bench] serving saturation: {sat}", file=sys.stderr, flush=True)
@@ -293,8 +298,12 @@ def measure_serving(dev, args) -> dict:
     plain_done = sv.run_plain(sub, probs, dev)
     plain_busy = sum(c.finished - c.admitted for c in plain_done)
     plain_rps = len(sub) / plain_busy
+    # matched final loss: the same requests' last-epoch loss on both paths
+    fused_loss = {c.rid: c.final_loss for c in sat_done}
+    loss_rel = max(abs(fused_loss[c.rid] - c.final_loss) / abs(c.final_loss) for c in plain_done)
     plain_sat = {"requests": len(sub), "throughput_req_per_s": round(plain_rps, 3),
-                 "mean_service_ms": round(1e3 * plain_busy / len(sub), 2)}
+                 "mean_service_ms": round(1e3 * plain_busy / len(sub), 2),
+                 "final_loss_max_rel_diff_vs_server": float(f"{loss_rel:.3g}")}
     print(f"[bench] plain per-request path: {plain_sat}", file=sys.stderr, flush=True)
 
     rates = []
@@ -311,7 +320,12 @@ def measure_serving(dev, args) -> dict:
     for r in rates:
         print(f"[bench] serving at {r['rate_req_per_s']} req/s: server {r['server']}"
               + (f", plain {r['plain']}" if r["plain"] else ""), file=sys.stderr, flush=True)
+    held = all(r["server"]["p99_ms"] <= P99_BOUND_MS for r in rates)
+    print(f"[bench] serving p99 bound {P99_BOUND_MS} ms at rates up to 0.9x saturation: "
+          f"{'held' if held else 'NOT held'}", file=sys.stderr, flush=True)
     return {
+        "p99_bound_ms": P99_BOUND_MS,
+        "p99_bound_held": held,
         "workload": "mixed stream: conventional 128/512 and 256/1024 (100-400 positions), Fourier 128^2/512^2 "
                     "(81|225 LEDs), near-field 512/1024 (64), coded 512^2 (25); 20 Adam epochs each",
         "requests": n,

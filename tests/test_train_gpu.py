@@ -149,3 +149,32 @@ def test_graph_replays_match_eager(gpu):
     for (le, se, pe), (lg, sg, pg) in zip(*runs):
         assert le == lg
         assert torch.equal(se, sg) and torch.equal(pe, pg)
+
+
+def test_cpu_workload_recipe_trains_like_the_reference_loop(gpu):
+    # the reference's CPU-runnable conventional case at its own sizes: object 512^2
+    # (smooth random amplitude/phase, sigma 4), probe 128^2 (disc r = 32, defocus
+    # exp(i pi r^2/400)), 15 x 15 raster, step 24 px, +-2 px jitter, Poisson 1e4
+    # photons; amplitude loss, Adam lr 1e-2, mini-batch 32, 10 epochs, seed 0,
+    # initial guesses of python/train_ptychography.py:495-498
+    from ptychox_amd.synthetic import conventional_dataset
+    from ptychox_amd.train import ConventionalReconstruction
+
+    d = conventional_dataset(512, 128, 32.0, 15, 24, 24, 2, 1e4, 4.0, gpu, seed=0)
+    gen = torch.Generator(device=gpu).manual_seed(0)
+    init_s = torch.rand((512, 512), device=gpu, generator=gen).to(torch.complex64)
+    init_p = (d.probe.abs() > 0).float().to(torch.complex64)
+    epochs, batch, lr = 10, 32, 1e-2
+    g1 = torch.Generator(device=gpu).manual_seed(0)
+    rec = ConventionalReconstruction(init_s, init_p, d.positions, d.measured, batch, lr=lr, epochs=epochs,
+                                     generator=g1, graph=True)
+    res = rec.fit()
+    g2 = torch.Generator(device=gpu).manual_seed(0)
+    perms = [torch.randperm(d.positions.shape[0], generator=g2, device=gpu) for _ in range(epochs)]
+    _, _, l_ref = _torch_loop(init_s, init_p, d.positions, d.measured, perms, batch, lr, lr, epochs)
+    losses = [r.loss for r in res]
+    e = nrel(losses, l_ref)
+    print(f"CPU-workload recipe, 10 epochs: epoch losses {[round(x, 4) for x in losses]} vs reference loop "
+          f"{[round(x, 4) for x in l_ref]} (rel {e:.2e})", flush=True)
+    assert losses[-1] < losses[0]
+    assert e < 5e-3
