@@ -91,8 +91,10 @@ const char* ptx_timing_label(int k);
  * is compiled for 8 and for 16 complex elements per thread (the radix of its
  * in-register butterflies); this table picks one per pass kind and transform
  * length.  kind: 0 row passes, 1 row passes that loop over a chunk of views
- * (object / probe / mask accumulation), 2 column passes, 3 column passes that
- * loop over views.  elems: 8 or 16, 0 restores the built-in default.
+ * (object / probe accumulation), 2 column passes, 3 column passes that
+ * loop over views, 4 row passes that loop over views keeping several frames'
+ * values per thread (coded mask gradient).  elems: 8 or 16, 0 restores the
+ * built-in default.
  * Process-wide; set it before launching work (the autotuner in
  * ptychox_amd/autotune.py measures and fills it). */
 ptx_status ptx_tune_set(int kind, int64_t n, int elems);

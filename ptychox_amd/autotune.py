@@ -22,8 +22,9 @@ import torch
 
 from . import _lib
 
-PASS_ROW, PASS_ROW_LOOP, PASS_COL, PASS_COL_LOOP = 0, 1, 2, 3
-KIND_NAMES = {PASS_ROW: "row", PASS_ROW_LOOP: "row_loop", PASS_COL: "col", PASS_COL_LOOP: "col_loop"}
+PASS_ROW, PASS_ROW_LOOP, PASS_COL, PASS_COL_LOOP, PASS_VIEW_ROWS = 0, 1, 2, 3, 4
+KIND_NAMES = {PASS_ROW: "row", PASS_ROW_LOOP: "row_loop", PASS_COL: "col", PASS_COL_LOOP: "col_loop",
+              PASS_VIEW_ROWS: "view_rows"}
 CANDIDATES = (8, 16)
 
 
@@ -112,7 +113,7 @@ def keys_for_fpm(object_shape, pupil_shape) -> list[tuple[int, int]]:
 
 def keys_for_coded(shape) -> list[tuple[int, int]]:
     H, W = shape
-    return [(PASS_ROW, W), (PASS_ROW_LOOP, W), (PASS_COL, H), (PASS_COL_LOOP, H)]
+    return [(PASS_ROW, W), (PASS_VIEW_ROWS, W), (PASS_COL, H), (PASS_COL_LOOP, H)]
 
 
 _env = os.environ.get("PTYCHOX_TUNING")

@@ -45,18 +45,6 @@ inline void sum_parts2(const float2* part, int64_t nparts, int64_t n, float2* ou
     k_sum_parts2<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(part, nparts, n, out, accumulate, conj);
 }
 
-// out[b] = scale * sum_{k < nparts} part[b*nparts + k], in ascending k
-static __global__ void __launch_bounds__(256) k_loss_rows(const float* __restrict__ part, int64_t nparts, int64_t batch,
-                                                   float scale, float* __restrict__ out) {
-  const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (b >= batch) return;
-  float s = 0.0f;
-  for (int64_t k = 0; k < nparts; ++k) s += part[b * nparts + k];
-  out[b] = s * scale;
-}
-inline void loss_rows(const float* part, int64_t nparts, int64_t batch, float scale, float* out, hipStream_t s) {
-  if (batch > 0) k_loss_rows<<<dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, s>>>(part, nparts, batch, scale, out);
-}
 
 // Sum of x over the work-group (blockDim.x a power of two <= 1024); result valid in
 // every thread.  Shuffle offsets stay below blockDim.x so that work-groups smaller
@@ -73,6 +61,22 @@ __device__ __forceinline__ float block_sum(float x, float* red /* >= 16 floats o
   for (int w = 0; w < nwaves; ++w) s += red[w];  // fixed order
   lds_barrier();
   return s;
+}
+
+// out[b] = scale * sum_{k < nparts} part[b*nparts + k]: one work-group per b,
+// each thread summing parts k = thread, thread + 256, ... in ascending order,
+// then the fixed-order work-group sum (deterministic)
+static __global__ void __launch_bounds__(256) k_loss_rows(const float* __restrict__ part, int64_t nparts,
+                                                   float scale, float* __restrict__ out) {
+  __shared__ float red[16];
+  const int64_t b = blockIdx.x;
+  float s = 0.0f;
+  for (int64_t k = threadIdx.x; k < nparts; k += 256) s += part[b * nparts + k];
+  const float tot = block_sum(s, red);
+  if (threadIdx.x == 0) out[b] = tot * scale;
+}
+inline void loss_rows(const float* part, int64_t nparts, int64_t batch, float scale, float* out, hipStream_t s) {
+  if (batch > 0) k_loss_rows<<<dim3((unsigned)batch), dim3(256), 0, s>>>(part, nparts, scale, out);
 }
 
 // ---------------------------------------------------------------- sorted-segment scatter

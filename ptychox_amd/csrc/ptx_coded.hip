@@ -21,6 +21,7 @@
 // then fixed-order partial sums and the object-size inverse FFT.  Both shared
 // gradients accumulate in registers over the chunk: no atomics anywhere, so
 // the operator is deterministic.
+#include "buffer_io.hpp"
 #include "common_kernels.hpp"
 #include "passes.hpp"
 #include "ptx_internal.hpp"
@@ -264,9 +265,15 @@ template <int N, int E>
 __global__ void __launch_bounds__(256)
     k_coded_y_rows(float2* __restrict__ tb, const float2* __restrict__ xfwd, const float2* __restrict__ mask,
                    int64_t H, int64_t B, int chunk, int ngroups, float inv_n, float2* __restrict__ gc_part) {
+  // Views of a chunk are pipelined: the next view's row (T) is loaded while
+  // this view's transforms run, and its X row once this view's X is consumed;
+  // buffer addressing (one VGPR per lane), LDS-only barriers and LDS twiddles
+  // keep those loads in flight.
   using Cf = RowCfg<N, E>;
   constexpr int T = Cf::T, S = Cf::S;
   __shared__ float2 lds[S * lds_len<N>()];
+  __shared__ float2 s_tw[N];
+  load_lds_twiddles<N>(s_tw);
   const int seq = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const int group = blockIdx.x % ngroups;
@@ -275,31 +282,52 @@ __global__ void __launch_bounds__(256)
   const bool yv = y < H;
   const int64_t b0 = (int64_t)ch * chunk;
   const int64_t b1 = b0 + chunk < B ? b0 + chunk : B;
-  float2 cm[E], acc[E];
+  const uint32_t fbytes = (uint32_t)(H * N * 8);
+  const uint32_t lane = (uint32_t)(((yv ? y : 0) * N + t) * 8);  // rows past H read row 0, never stored
+  float2 cm[E], acc[E], nv[E], nx[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     cm[q] = yv ? mask[y * N + t + q * T] : c_zero();
     acc[q] = c_zero();
   }
+  auto fetch_t = [&](int64_t b, float2 (&dv)[E]) {
+    const auto r = buffer_rsrc(tb + b * H * N, fbytes);
+#pragma unroll
+    for (int q = 0; q < E; ++q) dv[q] = buf_load_f2(r, lane, (uint32_t)(q * T * 8));
+  };
+  auto fetch_x = [&](int64_t b, float2 (&dx)[E]) {
+    const auto r = buffer_rsrc(xfwd + b * H * N, fbytes);
+#pragma unroll
+    for (int q = 0; q < E; ++q) dx[q] = buf_load_f2(r, lane, (uint32_t)(q * T * 8));
+  };
+  if (b0 < b1) {
+    fetch_t(b0, nv);
+    fetch_x(b0, nx);
+  }
   for (int64_t b = b0; b < b1; ++b) {
-    const int64_t row = b * H + y;
     float2 v[E];
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = yv ? tb[row * N + t + q * T] : c_zero();
-    fft_seq<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
+    for (int q = 0; q < E; ++q) v[q] = nv[q];
+    if (b + 1 < b1) fetch_t(b + 1, nv);
+    int tt = t;
+    asm volatile("" : "+v"(tt));  // keep the exchange addresses per view, not hoisted into registers
+    float2* sl = &lds[seq * lds_len<N>()];
+    fft_seq_tw<N, E, +1>(v, tt, sl, s_tw);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const float2 gy = c_scale(v[q], inv_n);
-      if (yv) acc[q] = c_add(acc[q], c_mulc(gy, xfwd[row * N + t + q * T]));
+      acc[q] = c_add(acc[q], c_mulc(gy, nx[q]));
       v[q] = c_mulc(gy, cm[q]);
     }
-    __syncthreads();
-    fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
+    if (b + 1 < b1) fetch_x(b + 1, nx);
+    lds_barrier();
+    fft_seq_tw<N, E, -1>(v, tt, sl, s_tw);
     if (yv) {
+      const auto r = buffer_rsrc(tb + b * H * N, fbytes);
 #pragma unroll
-      for (int q = 0; q < E; ++q) tb[row * N + t + q * T] = v[q];
+      for (int q = 0; q < E; ++q) buf_store_f2(v[q], r, lane, (uint32_t)(q * T * 8));
     }
-    __syncthreads();
+    lds_barrier();
   }
   if (yv) {
 #pragma unroll
@@ -307,7 +335,10 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Q4: per (column tile, chunk of views): column FFT; spectrum partial += conj(H1 R_b) v / N
+// Q4: per (column tile, chunk of views): column FFT; spectrum partial
+// += conj(H1 R_b) v / N, with the view-independent conj(H1)/N applied once
+// after the view loop.  Views are pipelined as in k_coded_y_rows (next view's
+// tile and row-ramp values loaded during this view's transform).
 template <int N, int E, int C>
 __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     k_coded_spec_cols(const float2* __restrict__ tb, const float2* __restrict__ h1, const float2* __restrict__ ex,
@@ -316,6 +347,8 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   using Cf = ColCfg<N, E>;
   constexpr int T = Cf::T;
   __shared__ float2 lds[C * lds_len<N>()];
+  __shared__ float2 s_tw[N];
+  load_lds_twiddles<N>(s_tw);
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
   const int64_t blk = xcd_paired_block(blockIdx.x, gridDim.x);
@@ -324,30 +357,44 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   const int64_t col = tile * C + c;
   const int64_t b0 = (int64_t)ch * chunk;
   const int64_t b1 = b0 + chunk < B ? b0 + chunk : B;
-  float2 hc[E], acc[E];
+  const uint32_t fbytes = (uint32_t)(N * width * 8);
+  const uint32_t lane = (uint32_t)((t * width + c) * 8);
+  const uint32_t col0 = (uint32_t)(tile * C * 8);
+  const uint32_t rstep = (uint32_t)(T * width * 8);
+  float2 acc[E], nv[E], ny[E], nrx;
 #pragma unroll
-  for (int q = 0; q < E; ++q) {
-    hc[q] = h1 ? h1[(t + q * T) * (int)width + (int)col] : make_float2(1.0f, 0.0f);  // null: H1 = 1
-    acc[q] = c_zero();
-  }
+  for (int q = 0; q < E; ++q) acc[q] = c_zero();
+  auto fetch = [&](int64_t b, float2 (&dv)[E], float2 (&dy)[E], float2& drx) {
+    drx = ex[b * width + col];
+#pragma unroll
+    for (int q = 0; q < E; ++q) dy[q] = ey[b * N + t + q * T];
+    const auto r = buffer_rsrc(tb + b * N * width, fbytes);
+#pragma unroll
+    for (int q = 0; q < E; ++q) dv[q] = buf_load_f2(r, lane, q * rstep + col0);
+  };
+  if (b0 < b1) fetch(b0, nv, ny, nrx);
   for (int64_t b = b0; b < b1; ++b) {
-    const float2* p = tb + b * N * width;  // frame base; the lane adds its column
-    float2 v[E];
-#pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = p[(t + q * T) * (int)width + (int)col];
-    fft_seq<N, E, -1>(v, t, &lds[c * lds_len<N>()]);
-    const float2 rx = ex[b * width + col];
+    float2 v[E], ry[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      const float2 r = c_mul(ey[b * N + t + q * T], rx);
-      // conj(H1 R) v / N
-      acc[q] = c_add(acc[q], c_scale(c_mulc(c_mulc(v[q], hc[q]), r), inv_n));
+      v[q] = nv[q];
+      ry[q] = ny[q];
     }
-    __syncthreads();
+    const float2 rx = nrx;
+    if (b + 1 < b1) fetch(b + 1, nv, ny, nrx);
+    int tt = t;
+    asm volatile("" : "+v"(tt));  // keep the exchange addresses per view, not hoisted into registers
+    fft_seq_tw<N, E, -1>(v, tt, &lds[c * lds_len<N>()], s_tw);
+#pragma unroll
+    for (int q = 0; q < E; ++q) acc[q] = c_add(acc[q], c_mulc(v[q], c_mul(ry[q], rx)));  // v conj(R_b)
+    lds_barrier();
   }
   float2* d = gs_part + (int64_t)ch * N * width;  // slice base; the lane adds its column
 #pragma unroll
-  for (int q = 0; q < E; ++q) d[(t + q * T) * (int)width + (int)col] = acc[q];
+  for (int q = 0; q < E; ++q) {
+    const float2 hc = h1 ? h1[(t + q * T) * (int)width + (int)col] : make_float2(1.0f, 0.0f);  // null: H1 = 1
+    d[(t + q * T) * (int)width + (int)col] = c_scale(c_mulc(acc[q], hc), inv_n);
+  }
 }
 
 // ---------------------------------------------------------------- host launchers
@@ -557,7 +604,7 @@ ptx_status check(const char* fn, int64_t H, int64_t W, int64_t B, int64_t ds) {
 
 Chunks plan(int64_t H, int64_t W, int64_t B) {
   Chunks c{};
-  dispatch_ne<RowChunkQuery>(PASS_ROW_LOOP, W, H, B, &c);
+  dispatch_ne<RowChunkQuery>(PASS_VIEW_ROWS, W, H, B, &c);
   dispatch_ne<ColChunkQuery>(PASS_COL_LOOP, H, W, B, &c);
   return c;
 }
@@ -591,7 +638,7 @@ ptx_status backward_from_t(const float2* xfwd, const float2* mask, const float2*
   const float inv_n = 1.0f / (float)(H * W);
   ptx_status st = dispatch_ne<TransferRun>(PASS_COL, H, ws.tb, t2, B, W, true, s);
   if (st != PTX_OK) return st;
-  if ((st = dispatch_ne<YRowsRun>(PASS_ROW_LOOP, W, ws.tb, xfwd, mask, H, B, c, inv_n, ws.gc_part, s)) != PTX_OK) return st;
+  if ((st = dispatch_ne<YRowsRun>(PASS_VIEW_ROWS, W, ws.tb, xfwd, mask, H, B, c, inv_n, ws.gc_part, s)) != PTX_OK) return st;
   if ((st = dispatch_ne<SpecGradColsRun>(PASS_COL_LOOP, H, ws.tb, t1, ws.ex, ws.ey, B, W, c, inv_n, ws.gs_part, s)) != PTX_OK)
     return st;
   sum_parts2(ws.gc_part, c.row_nchunks, H * W, gcoded, accumulate, s, matlab ? 1 : 0);

@@ -51,7 +51,9 @@ inline int64_t persistent_grid(int64_t ntiles, int per_cu) {
 void timing_begin(hipStream_t stream);
 
 // Pass kinds with their own elements-per-thread setting in the tuning table.
-enum PassKind { PASS_ROW = 0, PASS_ROW_LOOP = 1, PASS_COL = 2, PASS_COL_LOOP = 3, PASS_KINDS = 4 };
+// PASS_VIEW_ROWS: row passes that loop over views holding per-element state
+// of several frames in registers (the coded mask-gradient pass).
+enum PassKind { PASS_ROW = 0, PASS_ROW_LOOP = 1, PASS_COL = 2, PASS_COL_LOOP = 3, PASS_VIEW_ROWS = 4, PASS_KINDS = 5 };
 
 // Elements per thread (8 or 16) the tuning table holds for (kind, n); n a
 // supported power of two.  Defaults come from MI355X measurements (radix-8

@@ -20,7 +20,7 @@ SIZES = (16, 32, 64, 128, 256, 512, 1024, 2048)
 def _force(elems):
     from ptychox_amd import _lib
 
-    for kind in range(4):
+    for kind in range(5):
         for n in SIZES:
             _lib.call("ptx_tune_set", kind, n, elems)
 
