@@ -30,12 +30,24 @@ struct RowCfg {
   static constexpr int S = THREADS / T;  // rows per work-group
 };
 
+// LDS stride between the columns of a column tile (float2 units).  With C >= 16
+// columns the 16 lanes that share an LDS cycle all hold one row index and the
+// padded sequence stride lds_len (= 1 mod 32) spreads them over the banks.
+// With C = 8 (C = 4) two (four) row indices share those lanes, and an odd
+// stride puts column c + 1 of row t on the banks of column c of row t + 1;
+// an even stride (+1; +3 for C = 4) separates them -- from a bank model of
+// every exchange of the column transforms, confirmed by SQ_LDS_BANK_CONFLICT.
+template <int N, int C>
+__host__ __device__ constexpr int col_stride() {
+  return C >= 16 ? lds_len<N>() : C == 4 ? lds_len<N>() + 3 : lds_len<N>() + 1;
+}
+
 template <int N, int E = elems_for<N>()>
 struct ColCfg {
   static_assert(E <= N && N % E == 0, "E must divide N");
   static constexpr int T = N / E;
   // LDS budget per work-group ~72 KB (two work-groups per CU), <= 1024 threads
-  static constexpr int BY_LDS = 73728 / (8 * lds_len<N>());
+  static constexpr int BY_LDS = 73728 / (8 * (lds_len<N>() + 3));
   static constexpr int BY_THREADS = 1024 / T;
   static constexpr int CM0 = BY_LDS < BY_THREADS ? BY_LDS : BY_THREADS;
   static constexpr int CMAX = CM0 >= 16 ? 16 : CM0 >= 8 ? 8 : CM0 >= 4 ? 4 : 2;
@@ -73,7 +85,7 @@ __global__ void __launch_bounds__(256) k_rows(Op op, int64_t nrows) {
 
 template <int N, int E, int C, int DIR, class Op>
 __global__ void __launch_bounds__((ColCfg<N, E>::T * C)) k_cols(Op op, int64_t tiles_per_frame) {
-  __shared__ float2 lds[C * lds_len<N>()];
+  __shared__ float2 lds[C * col_stride<N, C>()];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
   const int64_t blk = xcd_paired_block(blockIdx.x, gridDim.x);
@@ -81,7 +93,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C)) k_cols(Op op, int64_t t
   const int64_t col = (blk % tiles_per_frame) * C + c;
   float2 v[E];
   op.template load<N, E>(frame, col, t, v);
-  fft_seq<N, E, DIR>(v, t, &lds[c * lds_len<N>()]);
+  fft_seq<N, E, DIR>(v, t, &lds[c * col_stride<N, C>()]);
   op.template store<N, E>(frame, col, t, v);
 }
 
@@ -119,7 +131,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
                     int64_t tiles_per_frame) {
   using Cf = ColCfg<N, E>;
   constexpr int T = Cf::T;
-  __shared__ float2 lds[C * lds_len<N>()];
+  __shared__ float2 lds[C * col_stride<N, C>()];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
   const int64_t blk = xcd_paired_block(blockIdx.x, gridDim.x);
@@ -129,14 +141,14 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   float2 v[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = p[(t + q * T) * (int)width + (int)col];
-  fft_seq<N, E, -1>(v, t, &lds[c * lds_len<N>()]);
+  fft_seq<N, E, -1>(v, t, &lds[c * col_stride<N, C>()]);
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const float2 tr = transfer[(t + q * T) * (int)width + (int)col];
     v[q] = CONJ ? c_mulc(v[q], tr) : c_mul(v[q], tr);
   }
   lds_barrier();  // second transform reuses the LDS region
-  fft_seq<N, E, +1>(v, t, &lds[c * lds_len<N>()]);
+  fft_seq<N, E, +1>(v, t, &lds[c * col_stride<N, C>()]);
 #pragma unroll
   for (int q = 0; q < E; ++q) p[(t + q * T) * (int)width + (int)col] = v[q];
 }

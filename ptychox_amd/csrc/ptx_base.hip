@@ -89,7 +89,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
                      int64_t tiles_per_frame, int64_t ntiles, float gscale) {
   using Cf = ColCfg<N, E>;
   constexpr int T = Cf::T;
-  __shared__ float2 lds[C * lds_len<N>()];
+  __shared__ float2 lds[C * col_stride<N, C>()];
   __shared__ float red[16];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     // instead of letting them be hoisted out of the loop and held in registers
     int tt = t, cc = c;
     asm volatile("" : "+v"(tt), "+v"(cc));
-    float2* seq_lds = &lds[cc * lds_len<N>()];
+    float2* seq_lds = &lds[cc * col_stride<N, C>()];
     fft_seq_tw<N, E, -1>(v, tt, seq_lds, s_tw);
     float lsum = 0.0f;
 #pragma unroll

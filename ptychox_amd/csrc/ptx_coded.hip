@@ -346,7 +346,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
                       float2* __restrict__ gs_part) {
   using Cf = ColCfg<N, E>;
   constexpr int T = Cf::T;
-  __shared__ float2 lds[C * lds_len<N>()];
+  __shared__ float2 lds[C * col_stride<N, C>()];
   __shared__ float2 s_tw[N];
   load_lds_twiddles<N>(s_tw);
   const int c = threadIdx.x % C;
@@ -384,7 +384,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     if (b + 1 < b1) fetch(b + 1, nv, ny, nrx);
     int tt = t;
     asm volatile("" : "+v"(tt));  // keep the exchange addresses per view, not hoisted into registers
-    fft_seq_tw<N, E, -1>(v, tt, &lds[c * lds_len<N>()], s_tw);
+    fft_seq_tw<N, E, -1>(v, tt, &lds[c * col_stride<N, C>()], s_tw);
 #pragma unroll
     for (int q = 0; q < E; ++q) acc[q] = c_add(acc[q], c_mulc(v[q], c_mul(ry[q], rx)));  // v conj(R_b)
     lds_barrier();

@@ -124,7 +124,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
                     float* __restrict__ loss_part, int64_t width, int64_t tiles_per_frame, float gscale, float inv_n) {
   using Cf = ColCfg<N, E>;
   constexpr int T = Cf::T;
-  __shared__ float2 lds[C * lds_len<N>()];
+  __shared__ float2 lds[C * col_stride<N, C>()];
   __shared__ float red[16];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
@@ -136,7 +136,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   float2 v[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = tb[off + ((t + q * T) * (int)width + (int)col)];
-  fft_seq<N, E, +1>(v, t, &lds[c * lds_len<N>()]);
+  fft_seq<N, E, +1>(v, t, &lds[c * col_stride<N, C>()]);
   float lsum = 0.0f;
 #pragma unroll
   for (int q = 0; q < E; ++q) {
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   }
   const float tot = block_sum(lsum, red);
   if (threadIdx.x == 0) loss_part[frame * tiles_per_frame + tile] = tot;
-  fft_seq<N, E, -1>(v, t, &lds[c * lds_len<N>()]);
+  fft_seq<N, E, -1>(v, t, &lds[c * col_stride<N, C>()]);
 #pragma unroll
   for (int q = 0; q < E; ++q) tb[off + ((t + q * T) * (int)width + (int)col)] = v[q];
 }
