@@ -28,13 +28,23 @@ static __global__ void __launch_bounds__(256) k_sum_parts2(const float2* __restr
                                                     float2* __restrict__ out, int accumulate, int conj) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
+  // eight slices' loads issued before their additions, which stay in ascending c
+  auto sum_into = [&](float2 s) {
+    int64_t c = 0;
+    for (; c + 8 <= nparts; c += 8) {
+      float2 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = part[(c + u) * n + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s = c_add(s, v[u]);
+    }
+    for (; c < nparts; ++c) s = c_add(s, part[c * n + i]);
+    return s;
+  };
   if (!conj) {
-    float2 s = accumulate ? out[i] : c_zero();
-    for (int64_t c = 0; c < nparts; ++c) s = c_add(s, part[c * n + i]);
-    out[i] = s;
+    out[i] = sum_into(accumulate ? out[i] : c_zero());
   } else {
-    float2 t = c_zero();
-    for (int64_t c = 0; c < nparts; ++c) t = c_add(t, part[c * n + i]);
+    float2 t = sum_into(c_zero());
     t.y = -t.y;
     out[i] = accumulate ? c_add(out[i], t) : t;
   }
@@ -135,6 +145,10 @@ static __global__ void __launch_bounds__(256) k_tile_lists(const int64_t* __rest
 // views in LDS, then walks them two at a time so that sixteen independent
 // 8-byte loads per thread are in flight (the per-view latency was the limit).
 constexpr int kViewsPerPass = 256;
+// Views gathered per iteration: their 8-row loads are all in flight before the
+// ordered additions.  Heavily overlapped tiles (the Fourier spectrum near DC,
+// near-field windows) sum tens of views in one work-group, which this bounds.
+constexpr int kGatherUnroll = 4;
 
 static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __restrict__ contrib, int h, int w,
                                                             const int64_t* __restrict__ pos, int64_t batch,
@@ -168,13 +182,13 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
       s_ox[k] = (int)pmod(pos[2 * b] + off_x, W);
     }
     __syncthreads();
-    for (int k = 0; k < m; k += 2) {
-      float2 t0[RPT], t1[RPT];
-      bool u0[RPT], u1[RPT];
+    for (int k = 0; k < m; k += kGatherUnroll) {
+      float2 tv_all[kGatherUnroll][RPT];
+      bool uv_all[kGatherUnroll][RPT];
 #pragma unroll
-      for (int v = 0; v < 2; ++v) {
-        float2* tv = v == 0 ? t0 : t1;
-        bool* uv = v == 0 ? u0 : u1;
+      for (int v = 0; v < kGatherUnroll; ++v) {
+        float2* tv = tv_all[v];
+        bool* uv = uv_all[v];
         const int kk = k + v;
         const bool have = kk < m;
         const int64_t b = have ? s_b[kk] : 0;
@@ -194,10 +208,10 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
         }
       }
 #pragma unroll
-      for (int j = 0; j < RPT; ++j) {
-        if (u0[j]) acc[j] = c_add(acc[j], t0[j]);  // ascending view order
-        if (u1[j]) acc[j] = c_add(acc[j], t1[j]);
-      }
+      for (int v = 0; v < kGatherUnroll; ++v)
+#pragma unroll
+        for (int j = 0; j < RPT; ++j)
+          if (uv_all[v][j]) acc[j] = c_add(acc[j], tv_all[v][j]);  // ascending view order
     }
   }
   if (!xin) return;
