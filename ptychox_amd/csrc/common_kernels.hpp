@@ -4,6 +4,7 @@
 #pragma once
 
 #include "fft_core.hpp"
+#include "ptx_internal.hpp"
 
 namespace ptx {
 
@@ -141,28 +142,106 @@ static __global__ void __launch_bounds__(256) k_tile_lists(const int64_t* __rest
 
 // grad[Y, X] (+)= sum over listed views b (ascending) of contrib[b, cy, cx] with
 // (cy, cx) = ((Y - oy_b) mod H, (X - ox_b) mod W) inside the h x w window.
-// The work-group first stages (b, oy_b, ox_b) of up to kViewsPerPass listed
-// views in LDS, then walks them two at a time so that sixteen independent
-// 8-byte loads per thread are in flight (the per-view latency was the limit).
-constexpr int kViewsPerPass = 256;
+//
+// Load balance: a tile's ascending view list is cut into segments of
+// kSegViews views, one work-group per segment (the plan below enumerates
+// them), so heavily overlapped tiles -- the Fourier spectrum near DC, the
+// near-field object under the probe -- spread over many work-groups.  Segment
+// 0 writes the tile's output, segments 1.. write partial tiles, and the
+// combine pass adds those to the output in segment order: a fixed summation
+// order, independent of scheduling.
+constexpr int kSegViews = 8;
 // Views gathered per iteration: their 8-row loads are all in flight before the
-// ordered additions.  Heavily overlapped tiles (the Fourier spectrum near DC,
-// near-field windows) sum tens of views in one work-group, which this bounds.
+// ordered additions.
 constexpr int kGatherUnroll = 4;
+constexpr int kTilePx = kTileH * kTileW;
 
+// Work items and partial-tile slots a gather can need: every tile has at
+// least one segment, and the segments beyond the first are bounded by the
+// list entries divided by kSegViews.  A view's window meets at most
+// h/16 + 2 tile rows (wrap-around on a grid of whole tiles included) and the
+// list test can add the grid's last, partial tile row; likewise for columns,
+// hence (h/16 + 3) x (w/128 + 3) list entries per view.
+struct GatherBounds {
+  int64_t items, slots;
+};
+inline GatherBounds gather_bounds(int64_t ntiles, int64_t h, int64_t w, int64_t batch) {
+  int64_t per_view = (h / kTileH + 3) * (w / kTileW + 3);
+  if (per_view > ntiles) per_view = ntiles;
+  const int64_t slots = (batch * per_view + kSegViews - 1) / kSegViews;
+  return GatherBounds{ntiles + slots, slots};
+}
+
+// One work-group: per tile nseg = max(1, ceil(count / kSegViews)); exclusive
+// scans give each tile its first work item (seg_off) and its first partial
+// slot (slot_off); item_tile[i] names the tile of work item i; *n_items the total.
+static __global__ void __launch_bounds__(1024)
+    k_gather_plan(const int* __restrict__ counts, int64_t ntiles, int* __restrict__ seg_off,
+                  int* __restrict__ slot_off, int* __restrict__ item_tile, int* __restrict__ n_items) {
+  __shared__ int s_seg[1024], s_slot[1024];
+  int carry_seg = 0, carry_slot = 0;
+  for (int64_t base = 0; base < ntiles; base += 1024) {
+    const int64_t t = base + threadIdx.x;
+    const int c = t < ntiles ? counts[t] : 0;
+    const int nseg = t < ntiles ? (c > kSegViews ? (c + kSegViews - 1) / kSegViews : 1) : 0;
+    s_seg[threadIdx.x] = nseg;
+    s_slot[threadIdx.x] = nseg > 0 ? nseg - 1 : 0;
+    __syncthreads();
+    // inclusive Hillis-Steele scans of both counts
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int a = threadIdx.x >= (unsigned)off ? s_seg[threadIdx.x - off] : 0;
+      const int b = threadIdx.x >= (unsigned)off ? s_slot[threadIdx.x - off] : 0;
+      __syncthreads();
+      s_seg[threadIdx.x] += a;
+      s_slot[threadIdx.x] += b;
+      __syncthreads();
+    }
+    if (t < ntiles) {
+      const int first = carry_seg + s_seg[threadIdx.x] - nseg;
+      seg_off[t] = first;
+      slot_off[t] = carry_slot + s_slot[threadIdx.x] - (nseg - 1);
+      for (int k = 0; k < nseg; ++k) item_tile[first + k] = (int)t;
+    }
+    carry_seg += s_seg[1023];
+    carry_slot += s_slot[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *n_items = carry_seg;
+}
+
+// SEG: one work item per segment (k_gather_plan's enumeration); otherwise one
+// work item per tile over its whole list, staged through LDS kListChunk views
+// at a time.
+constexpr int kListChunk = 64;
+template <bool SEG>
 static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __restrict__ contrib, int h, int w,
                                                             const int64_t* __restrict__ pos, int64_t batch,
                                                             int64_t off_y, int64_t off_x,
                                                             const int* __restrict__ lists,
                                                             const int* __restrict__ counts, int64_t tiles_x,
+                                                            const int* __restrict__ seg_off,
+                                                            const int* __restrict__ slot_off,
+                                                            const int* __restrict__ item_tile,
+                                                            const int* __restrict__ n_items,
+                                                            float2* __restrict__ partial,
                                                             float2* __restrict__ grad, int64_t H, int64_t W,
                                                             int accumulate) {
-  constexpr int RPT = kTileH * kTileW / 256;  // rows per thread
+  constexpr int RPT = kTilePx / 256;  // rows per thread
   constexpr int RSTEP = 256 / kTileW;
-  __shared__ int s_b[kViewsPerPass];
-  __shared__ int s_oy[kViewsPerPass];
-  __shared__ int s_ox[kViewsPerPass];
-  const int64_t tile = blockIdx.x;
+  constexpr int CHUNK = SEG ? kSegViews : kListChunk;
+  __shared__ int s_b[CHUNK];
+  __shared__ int s_oy[CHUNK];
+  __shared__ int s_ox[CHUNK];
+  int64_t tile;
+  int seg = 0;
+  if (SEG) {
+    const int item = blockIdx.x;
+    if (item >= *n_items) return;  // the grid is the host's upper bound
+    tile = item_tile[item];
+    seg = item - seg_off[tile];
+  } else {
+    tile = blockIdx.x;
+  }
   const int64_t ty0 = (tile / tiles_x) * kTileH, tx0 = (tile % tiles_x) * kTileW;
   const int64_t X = tx0 + (threadIdx.x % kTileW);
   const int64_t Y0 = ty0 + threadIdx.x / kTileW;
@@ -171,15 +250,17 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
 #pragma unroll
   for (int j = 0; j < RPT; ++j) acc[j] = c_zero();
   const int n = counts[tile];
+  const int vbeg = seg * kSegViews;
+  const int vend = SEG ? (n - vbeg < kSegViews ? n : vbeg + kSegViews) : n;
   const int* list = lists + tile * batch;
-  for (int base = 0; base < n; base += kViewsPerPass) {
-    const int m = n - base < kViewsPerPass ? n - base : kViewsPerPass;
-    __syncthreads();  // previous pass finished reading the staged views
-    for (int k = threadIdx.x; k < m; k += 256) {
-      const int64_t b = list[base + k];
-      s_b[k] = (int)b;
-      s_oy[k] = (int)pmod(pos[2 * b + 1] + off_y, H);
-      s_ox[k] = (int)pmod(pos[2 * b] + off_x, W);
+  for (int c0 = vbeg; c0 < vend; c0 += CHUNK) {
+    const int m = vend - c0 < CHUNK ? vend - c0 : CHUNK;
+    if (c0 > vbeg) __syncthreads();  // the previous chunk is consumed
+    if (threadIdx.x < (unsigned)m) {
+      const int64_t b = list[c0 + threadIdx.x];
+      s_b[threadIdx.x] = (int)b;
+      s_oy[threadIdx.x] = (int)pmod(pos[2 * b + 1] + off_y, H);
+      s_ox[threadIdx.x] = (int)pmod(pos[2 * b] + off_x, W);
     }
     __syncthreads();
     for (int k = 0; k < m; k += kGatherUnroll) {
@@ -214,6 +295,12 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
           if (uv_all[v][j]) acc[j] = c_add(acc[j], tv_all[v][j]);  // ascending view order
     }
   }
+  if (SEG && seg > 0) {  // a partial tile, added by the combine pass
+    float2* pt = partial + (int64_t)(slot_off[tile] + seg - 1) * kTilePx;
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) pt[j * 256 + threadIdx.x] = acc[j];
+    return;
+  }
   if (!xin) return;
 #pragma unroll
   for (int j = 0; j < RPT; ++j) {
@@ -223,6 +310,38 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
       *d = accumulate ? c_add(*d, acc[j]) : acc[j];
     }
   }
+}
+
+// grad += the partial tiles of segments 1.. of every tile, in segment order.
+// Work-group (tile, j) adds pixel chunk j (256 pixels) of the tile, eight
+// partial tiles' loads in flight per step.
+constexpr int kCombineBatch = 8;
+static __global__ void __launch_bounds__(256) k_gather_combine(const int* __restrict__ counts,
+                                                               const int* __restrict__ slot_off, int64_t tiles_x,
+                                                               const float2* __restrict__ partial,
+                                                               float2* __restrict__ grad, int64_t H, int64_t W) {
+  constexpr int RPT = kTilePx / 256;
+  constexpr int RSTEP = 256 / kTileW;
+  const int64_t tile = blockIdx.x / RPT;
+  const int j = blockIdx.x % RPT;
+  const int n = counts[tile];
+  const int nseg = n > kSegViews ? (n + kSegViews - 1) / kSegViews : 1;
+  if (nseg < 2) return;
+  const int64_t ty0 = (tile / tiles_x) * kTileH, tx0 = (tile % tiles_x) * kTileW;
+  const int64_t X = tx0 + (threadIdx.x % kTileW);
+  const int64_t Y = ty0 + threadIdx.x / kTileW + j * RSTEP;
+  if (X >= W || Y >= H) return;
+  const float2* pt = partial + (int64_t)slot_off[tile] * kTilePx + j * 256 + threadIdx.x;
+  float2 s = grad[Y * W + X];
+  for (int sg = 0; sg < nseg - 1; sg += kCombineBatch) {
+    float2 v[kCombineBatch];
+#pragma unroll
+    for (int k = 0; k < kCombineBatch; ++k) v[k] = sg + k < nseg - 1 ? pt[(int64_t)(sg + k) * kTilePx] : c_zero();
+#pragma unroll
+    for (int k = 0; k < kCombineBatch; ++k)
+      if (sg + k < nseg - 1) s = c_add(s, v[k]);
+  }
+  grad[Y * W + X] = s;
 }
 
 struct TileGrid {
@@ -246,5 +365,71 @@ struct Carver {
     return p;
   }
 };
+
+// Segments pay for the plan and combine passes once the windows overlap by
+// at least a segment's worth of views on average.
+inline bool gather_segmented(int64_t h, int64_t w, int64_t B, int64_t H, int64_t W) {
+  return B * h * w >= (int64_t)kSegViews * H * W;
+}
+
+// Scratch of the deterministic gradient gather: per-tile view lists, the
+// segment plan and the partial tiles of segments beyond the first.
+struct GatherScratch {
+  int* lists;      // [ntiles, B]
+  int* counts;     // [ntiles]
+  int* seg_off;    // [ntiles]
+  int* slot_off;   // [ntiles]
+  int* item_tile;  // [bound on work items]
+  int* n_items;    // [1]
+  float2* partial; // [bound on partial slots, kTilePx]
+};
+
+inline GatherScratch carve_gather(Carver& cv, int64_t H, int64_t W, int64_t h, int64_t w, int64_t B) {
+  const TileGrid tg = tile_grid(H, W);
+  const GatherBounds gb = gather_bounds(tg.count(), h, w, B);
+  GatherScratch g{};
+  g.lists = cv.take<int>((size_t)(tg.count() * B));
+  g.counts = cv.take<int>((size_t)tg.count());
+  if (!gather_segmented(h, w, B, H, W)) return g;
+  g.seg_off = cv.take<int>((size_t)tg.count());
+  g.slot_off = cv.take<int>((size_t)tg.count());
+  g.item_tile = cv.take<int>((size_t)gb.items);
+  g.n_items = cv.take<int>(1);
+  g.partial = cv.take<float2>((size_t)(gb.slots * kTilePx));
+  return g;
+}
+
+// grad (+)= the windows' contributions (contrib [B, h, w], window corners
+// pos + (off_x, off_y), wrapping modulo (H, W)): tile lists, segment plan,
+// segmented gather, combine.
+inline ptx_status run_tile_gather(const GatherScratch& g, const float2* contrib, int64_t h, int64_t w,
+                                  const int64_t* pos, int64_t B, int64_t off_y, int64_t off_x, float2* grad,
+                                  int64_t H, int64_t W, int accumulate, hipStream_t s) {
+  const TileGrid tg = tile_grid(H, W);
+  ptx_status st;
+  k_tile_lists<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(pos, B, (int)h, (int)w, H, W, off_y, off_x,
+                                                                 tg.tiles_x, g.lists, g.counts);
+  if ((st = check_launch("tile lists", s)) != PTX_OK) return st;
+  if (!gather_segmented(h, w, B, H, W)) {
+    k_tile_gather<false><<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(
+        contrib, (int)h, (int)w, pos, B, off_y, off_x, g.lists, g.counts, tg.tiles_x, nullptr, nullptr, nullptr,
+        nullptr, nullptr, grad, H, W, accumulate);
+    return check_launch("tile gather", s);
+  }
+  const GatherBounds gb = gather_bounds(tg.count(), h, w, B);
+  k_gather_plan<<<dim3(1), dim3(1024), 0, s>>>(g.counts, tg.count(), g.seg_off, g.slot_off, g.item_tile, g.n_items);
+  if ((st = check_launch("gather plan", s)) != PTX_OK) return st;
+  k_tile_gather<true><<<dim3((unsigned)gb.items), dim3(256), 0, s>>>(
+      contrib, (int)h, (int)w, pos, B, off_y, off_x, g.lists, g.counts, tg.tiles_x, g.seg_off, g.slot_off,
+      g.item_tile, g.n_items, g.partial, grad, H, W, accumulate);
+  if ((st = check_launch("tile gather", s)) != PTX_OK) return st;
+  if (gb.slots > 0) {
+    k_gather_combine<<<dim3((unsigned)(tg.count() * (kTilePx / 256))), dim3(256), 0, s>>>(g.counts, g.slot_off,
+                                                                                          tg.tiles_x, g.partial,
+                                                                                          grad, H, W);
+    if ((st = check_launch("gather combine", s)) != PTX_OK) return st;
+  }
+  return PTX_OK;
+}
 
 }  // namespace ptx

@@ -347,8 +347,7 @@ struct FpmWorkspace {
   float2* tb;        // [B, h, w]
   float2* gq_part;   // [nchunks, h, w]
   float* loss_part;  // [B, col tiles] (step only)
-  int* lists;        // [ntiles, B] (deterministic)
-  int* counts;
+  GatherScratch gather;  // deterministic only
   size_t bytes;
 };
 
@@ -361,11 +360,7 @@ FpmWorkspace carve(void* base, int64_t H, int64_t W, int64_t h, int64_t w, int64
   ws.tb = cv.take<float2>((size_t)(B * h * w));
   ws.gq_part = cv.take<float2>((size_t)(p.nchunks * h * w));
   ws.loss_part = step ? cv.take<float>((size_t)(B * tiles)) : nullptr;
-  if (use_segments(flags)) {
-    const TileGrid tg = tile_grid(H, W);
-    ws.lists = cv.take<int>((size_t)(tg.count() * B));
-    ws.counts = cv.take<int>((size_t)tg.count());
-  }
+  if (use_segments(flags)) ws.gather = carve_gather(cv, H, W, h, w, B);
   ws.bytes = cv.used;
   return ws;
 }
@@ -414,13 +409,7 @@ ptx_status fpm_tail(const FpmWorkspace& ws, const Plan& p, const float2* spec, i
   sum_parts2(ws.gq_part, p.nchunks, h * w, gpupil, acc ? 1 : 0, s);
   if ((st = check_launch("pupil partial sum", s)) != PTX_OK) return st;
   if (det) {
-    const TileGrid tg = tile_grid(H, W);
-    k_tile_lists<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(led, B, (int)h, (int)w, H, W, H / 2, W / 2,
-                                                                   tg.tiles_x, ws.lists, ws.counts);
-    if ((st = check_launch("tile lists", s)) != PTX_OK) return st;
-    k_tile_gather<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(ws.tb, (int)h, (int)w, led, B, H / 2, W / 2,
-                                                                    ws.lists, ws.counts, tg.tiles_x, ws.gspec, H, W, 0);
-    if ((st = check_launch("tile gather", s)) != PTX_OK) return st;
+    if ((st = run_tile_gather(ws.gather, ws.tb, h, w, led, B, H / 2, W / 2, ws.gspec, H, W, 0, s)) != PTX_OK) return st;
   }
   st = dispatch_ne<ObjFftRows>(PASS_ROW, W, ws.gspec, ws.gspec, H, +1, s);
   if (st != PTX_OK) return st;

@@ -163,8 +163,7 @@ struct ScanWorkspace {
   float2* tb;        // [B, h, w]
   float2* gp_part;   // [nchunks, h, w]
   float* loss_part;  // [B, col tiles]   (step only)
-  int* lists;        // [ntiles, B]       (deterministic only)
-  int* counts;       // [ntiles]
+  GatherScratch gather;  // deterministic only
   int64_t* pos_batch;  // [B, 2] gathered positions (indexed step only)
   int* bad_index;      // set when an order entry is out of range (indexed step only)
   size_t bytes;
@@ -177,11 +176,7 @@ ScanWorkspace carve_scan(void* base, int64_t H, int64_t W, int64_t h, int64_t w,
   ws.tb = cv.take<float2>((size_t)(B * h * w));
   ws.gp_part = cv.take<float2>((size_t)(sp.nchunks * h * w));
   ws.loss_part = step ? cv.take<float>((size_t)(B * col_tiles)) : nullptr;
-  if (use_segments(flags)) {
-    const TileGrid tg = tile_grid(H, W);
-    ws.lists = cv.take<int>((size_t)(tg.count() * B));
-    ws.counts = cv.take<int>((size_t)tg.count());
-  }
+  if (use_segments(flags)) ws.gather = carve_gather(cv, H, W, h, w, B);
   if (indexed) {
     ws.pos_batch = cv.take<int64_t>((size_t)(2 * B));
     ws.bad_index = reinterpret_cast<int*>(cv.take<float2>(1));  // zeroed with the float2 fill kernel
@@ -226,13 +221,7 @@ ptx_status scan_scatter(const ScanWorkspace& ws, const ScatterPlan& sp, const fl
   sum_parts2(ws.gp_part, sp.nchunks, h * w, gp, acc ? 1 : 0, s);
   if ((st = check_launch("probe partial sum", s)) != PTX_OK) return st;
   if (det) {
-    const TileGrid tg = tile_grid(H, W);
-    k_tile_lists<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(pos, B, (int)h, (int)w, H, W, 0, 0, tg.tiles_x,
-                                                                   ws.lists, ws.counts);
-    if ((st = check_launch("tile lists", s)) != PTX_OK) return st;
-    k_tile_gather<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(ws.tb, (int)h, (int)w, pos, B, 0, 0, ws.lists,
-                                                                    ws.counts, tg.tiles_x, gs, H, W, acc ? 1 : 0);
-    if ((st = check_launch("tile gather", s)) != PTX_OK) return st;
+    if ((st = run_tile_gather(ws.gather, ws.tb, h, w, pos, B, 0, 0, gs, H, W, acc ? 1 : 0, s)) != PTX_OK) return st;
   }
   return PTX_OK;
 }
