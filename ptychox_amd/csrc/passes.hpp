@@ -68,6 +68,17 @@ __device__ __forceinline__ int64_t xcd_paired_block(unsigned b, unsigned nblocks
   return (int64_t)((b & ~15u) + 2u * (r & 7u) + (r >> 3));
 }
 
+// Frame and column tile of column-pass block blk.  Tiles per frame (frame
+// width / columns per tile, both powers of two) is a power of two, so this is
+// a shift and a mask instead of a 64-bit division.
+struct TileIdx {
+  int64_t frame, tile;
+};
+__device__ __forceinline__ TileIdx split_tile(int64_t blk, int64_t tiles_per_frame) {
+  const int sh = __builtin_ctzll((unsigned long long)tiles_per_frame);
+  return TileIdx{blk >> sh, blk & (tiles_per_frame - 1)};
+}
+
 template <int N, int E, int DIR, class Op>
 __global__ void __launch_bounds__(256) k_rows(Op op, int64_t nrows) {
   using Cf = RowCfg<N, E>;
@@ -89,8 +100,9 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C)) k_cols(Op op, int64_t t
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
   const int64_t blk = xcd_paired_block(blockIdx.x, gridDim.x);
-  const int64_t frame = blk / tiles_per_frame;
-  const int64_t col = (blk % tiles_per_frame) * C + c;
+  const TileIdx ti = split_tile(blk, tiles_per_frame);
+  const int64_t frame = ti.frame;
+  const int64_t col = ti.tile * C + c;
   float2 v[E];
   op.template load<N, E>(frame, col, t, v);
   fft_seq<N, E, DIR>(v, t, &lds[c * col_stride<N, C>()]);
@@ -135,8 +147,9 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
   const int64_t blk = xcd_paired_block(blockIdx.x, gridDim.x);
-  const int64_t frame = blk / tiles_per_frame;
-  const int64_t col = (blk % tiles_per_frame) * C + c;
+  const TileIdx ti = split_tile(blk, tiles_per_frame);
+  const int64_t frame = ti.frame;
+  const int64_t col = ti.tile * C + c;
   float2* p = tb + frame * N * width;
   float2 v[E];
 #pragma unroll

@@ -75,74 +75,85 @@ struct BaseGammaCols {
 
 // ---------------------------------------------------------------- C12: fused forward/loss/adjoint column pass
 // Persistent and software-pipelined: one resident work-group per CU walks the
-// column tiles blockIdx.x, + gridDim.x, ...; while it transforms tile k its
-// registers already hold the loads of tile k + gridDim.x (row-FFT output and
-// measured amplitudes), so the HBM traffic of the next tile overlaps the LDS
-// exchanges of this one.  Buffer accesses keep the addressing to one VGPR per
-// lane.  Per tile: column FFT, A = |Phi|, loss partial,
-// Gamma = gscale (A - Y) Phi / max(A, eps), column iFFT -- the same arithmetic
-// and the same per-tile loss partials as a one-tile-per-work-group launch.
+// column tiles blockIdx.x, + gridDim.x, ...; while it transforms tile k one of
+// two register sets already holds the loads of tile k + gridDim.x (row-FFT
+// output and measured amplitudes), so the HBM traffic of the next tile
+// overlaps the LDS exchanges of this one; the sets alternate, nothing is
+// copied between them.  Buffer accesses keep the addressing to one VGPR per
+// lane.  Per tile: column FFT, A = |Phi|, loss partials (one per wave),
+// Gamma = gscale (A - Y) Phi / max(A, eps), column iFFT.
+//
+// The exchanges alternate between two LDS regions, one barrier each: with an
+// even number of exchanges per transform, consecutive exchanges never share a
+// region, so no barrier separates the transforms or the tiles; with an odd
+// number one barrier follows each transform.
+template <int N, int E, int C>
+struct StepColsShape {
+  static constexpr int T = ColCfg<N, E>::T;
+  static constexpr int THREADS = T * C;
+  static constexpr int WAVES = (THREADS + 63) / 64;
+  static constexpr int EXCHANGES = FftShape<N, E>::STAGES - 1;
+  static constexpr size_t LDS_BYTES = (2 * (size_t)C * col_stride<N, C>() + N) * sizeof(float2);
+};
+
 template <int N, int E, int C>
 __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     k_base_step_cols(float2* __restrict__ tb, const float* __restrict__ measured, const int64_t* __restrict__ order,
                      int64_t count, float* __restrict__ amp_out, float* __restrict__ loss_part, int64_t width,
                      int64_t tiles_per_frame, int64_t ntiles, float gscale) {
-  using Cf = ColCfg<N, E>;
-  constexpr int T = Cf::T;
-  __shared__ float2 lds[C * col_stride<N, C>()];
-  __shared__ float red[16];
+  using Sh = StepColsShape<N, E, C>;
+  constexpr int T = Sh::T;
+  static_assert(Sh::LDS_BYTES <= 160 * 1024, "LDS budget");
+  __shared__ float2 lds0[C * col_stride<N, C>()];
+  __shared__ float2 lds1[C * col_stride<N, C>()];
+  __shared__ float2 s_tw[N];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
+  const int wave = threadIdx.x >> 6;
+  const bool lane0 = (threadIdx.x & 63) == 0;
   const int64_t fsz = (int64_t)N * width;               // elements per frame
   const uint32_t fbytes = (uint32_t)(fsz * 8);
   const uint32_t lane = (uint32_t)(t * (int)width + c);  // element offset inside a tile (row t, column c)
   const uint32_t rstep = (uint32_t)(T * width);          // elements from row t + q T to t + (q + 1) T
   const unsigned nt = (unsigned)ntiles;
-  __shared__ float2 s_tw[N];
   load_lds_twiddles<N>(s_tw);
-  float2 nv[E];
-  float nm[E];
   // loads of tile kk into (dv, dm); the measured frame of a view is the
-  // batch's own (order == null) or measured[order[frame]]
+  // batch's own (order == null) or measured[order[frame]].  A tile past the
+  // end gets zero-length buffers: its loads are issued but return zero
+  // without touching memory, so the loop's memory instructions are the same
+  // on every path and the compiler's wait counts (vmcnt) let the prefetch
+  // stay in flight instead of draining it at every tile.
+  // (tile and frame indices are compared as 32-bit scalars: a 64-bit ordered
+  // compare needs vector registers, whose reuse would wait on in-flight stores)
   auto fetch = [&](int64_t kk, float2 (&dv)[E], float (&dm)[E]) {
-    const int64_t blk = xcd_paired_block((unsigned)kk, nt);
-    const int64_t frame = blk / tiles_per_frame;
-    const uint32_t col0 = (uint32_t)((blk % tiles_per_frame) * C);
-    int64_t mframe = frame;
+    const bool live = (unsigned)kk < nt;
+    const TileIdx ti = split_tile(xcd_paired_block(live ? (unsigned)kk : 0u, nt), tiles_per_frame);
+    const uint32_t col0 = (uint32_t)(ti.tile * C);
+    int64_t mframe = ti.frame;
     if (order) {
-      mframe = order[frame];
-      if (mframe < 0 || mframe >= count) mframe = 0;  // flagged by the position gather
+      const uint64_t o = (uint64_t)order[ti.frame];
+      // out of range (flagged by the position gather): frame 0
+      mframe = ((o >> 32) == 0 && (uint32_t)o < (uint32_t)count) ? (int64_t)o : 0;
     }
-    const auto rt = buffer_rsrc(tb + frame * fsz, fbytes);
-    const auto rm = buffer_rsrc(measured + mframe * fsz, fbytes / 2);
+    const auto rt = buffer_rsrc(tb + ti.frame * fsz, live ? fbytes : 0u);
+    const auto rm = buffer_rsrc(measured + mframe * fsz, live ? fbytes / 2 : 0u);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       dv[q] = buf_load_f2(rt, lane * 8u, (q * rstep + col0) * 8u);
       dm[q] = buf_load_f1(rm, lane * 4u, (q * rstep + col0) * 4u);
     }
   };
-  int64_t k = blockIdx.x;
-  if (k < ntiles) fetch(k, nv, nm);
-  for (; k < ntiles; k += gridDim.x) {
-    float2 v[E];
-    float m[E];
-#pragma unroll
-    for (int q = 0; q < E; ++q) {
-      v[q] = nv[q];
-      m[q] = nm[q];
-    }
-    const int64_t kn = k + gridDim.x;
-    if (kn < ntiles) fetch(kn, nv, nm);  // in flight during this tile's transforms
-    const int64_t blk = xcd_paired_block((unsigned)k, nt);
-    const int64_t frame = blk / tiles_per_frame;
-    const int64_t tile = blk % tiles_per_frame;
-    const uint32_t col0 = (uint32_t)(tile * C);
+  auto process = [&](int64_t kk, float2 (&v)[E], float (&m)[E]) {
+    const TileIdx ti = split_tile(xcd_paired_block((unsigned)kk, nt), tiles_per_frame);
+    const uint32_t col0 = (uint32_t)(ti.tile * C);
     // the exchange addresses depend only on (t, c); recompute them per tile
     // instead of letting them be hoisted out of the loop and held in registers
     int tt = t, cc = c;
     asm volatile("" : "+v"(tt), "+v"(cc));
-    float2* seq_lds = &lds[cc * col_stride<N, C>()];
-    fft_seq_tw<N, E, -1>(v, tt, seq_lds, s_tw);
+    float2* l0 = &lds0[cc * col_stride<N, C>()];
+    float2* l1 = &lds1[cc * col_stride<N, C>()];
+    fft_stages<N, E, -1, true, 0, true>(v, tt, l0, l1, s_tw);
+    if constexpr (Sh::EXCHANGES % 2 == 1) lds_barrier();
     float lsum = 0.0f;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -152,18 +163,41 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
       v[q] = c_scale(v[q], gscale * d * fast_rcp(fmaxf(a, kFltEps)));
       m[q] = a;
     }
-    if (amp_out) {
-      const auto ra = buffer_rsrc(amp_out + frame * fsz, fbytes / 2);
+    // amplitudes (dropped by a zero-length buffer when not requested) and the
+    // wave's loss partial (lane 0; the other lanes' offsets fall outside the
+    // 4-byte buffer): stores without branches, for the same reason as above
+    const auto ra = buffer_rsrc(amp_out ? amp_out + ti.frame * fsz : measured, amp_out ? fbytes / 2 : 0u);
 #pragma unroll
-      for (int q = 0; q < E; ++q) buf_store_f1(m[q], ra, lane * 4u, (q * rstep + col0) * 4u);
-    }
-    const float tot = block_sum(lsum, red);  // contains the barrier that also orders the LDS reuse below
-    if (threadIdx.x == 0) loss_part[frame * tiles_per_frame + tile] = tot;
-    fft_seq_tw<N, E, +1>(v, tt, seq_lds, s_tw);
-    const auto rt = buffer_rsrc(tb + frame * fsz, fbytes);
+    for (int q = 0; q < E; ++q) buf_store_f1(m[q], ra, lane * 4u, (q * rstep + col0) * 4u);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      if (off < Sh::THREADS) lsum += __shfl_xor(lsum, off, 64);
+    buf_store_f1(lsum, buffer_rsrc(loss_part + (ti.frame * tiles_per_frame + ti.tile) * Sh::WAVES + wave, 4u),
+                 lane0 ? 0u : 64u, 0u);
+    fft_stages<N, E, +1, true, 0, true>(v, tt, l0, l1, s_tw);
+    if constexpr (Sh::EXCHANGES % 2 == 1) lds_barrier();
+    const auto rt = buffer_rsrc(tb + ti.frame * fsz, fbytes);
 #pragma unroll
     for (int q = 0; q < E; ++q) buf_store_f2(v[q], rt, lane * 8u, (q * rstep + col0) * 8u);
-    lds_barrier();  // the next tile's first exchange rewrites the LDS
+  };
+  // The first tile is peeled so that every path into the loop has issued the
+  // same memory instructions (two tiles' loads, one tile's stores): the wait
+  // for a tile's loads then never has to cover the previous tile's stores.
+  float2 va[E], vb[E];
+  float ma[E], mb[E];
+  int64_t k = blockIdx.x;  // < ntiles: the grid never exceeds the tile count
+  fetch(k, va, ma);
+  fetch(k + gridDim.x, vb, mb);  // in flight during this tile's transforms
+  process(k, va, ma);
+  k += gridDim.x;
+  while ((unsigned)k < nt) {
+    fetch(k + gridDim.x, va, ma);
+    process(k, vb, mb);
+    k += gridDim.x;
+    if ((unsigned)k >= nt) break;
+    fetch(k + gridDim.x, vb, mb);
+    process(k, va, ma);
+    k += gridDim.x;
   }
 }
 
@@ -185,24 +219,55 @@ struct GammaColsRun {
 };
 
 
+// Columns per work-group of k_base_step_cols: ColCfg's CMAX, or 8 for frames
+// narrower than 16 columns.
+template <int N, int E>
+inline int step_cols_c(int64_t width) {
+  using Cf = ColCfg<N, E>;
+  return (Cf::CMAX > 8 && width < Cf::CMAX) ? 8 : Cf::CMAX;
+}
+
+// loss partials per frame written by k_base_step_cols: one per wave of each column tile
+template <int N, int E>
+struct StepLossPartsQuery {
+  static ptx_status run(int64_t width, int64_t* parts) {
+    using Cf = ColCfg<N, E>;
+    const int c = step_cols_c<N, E>(width);
+    *parts = (width / c) * ((Cf::T * c + 63) / 64);
+    return PTX_OK;
+  }
+};
+
+template <int N, int E, int C>
+inline void launch_step_cols(float2* tb, const float* measured, const int64_t* order, int64_t count, float* amp,
+                             float* loss_part, int64_t frames, int64_t width, float gscale, hipStream_t s) {
+  using Sh = StepColsShape<N, E, C>;
+  const int64_t tiles = width / C;
+  int per_cu = 1024 / (Sh::T * C);
+  const int by_lds = (int)((160 * 1024) / Sh::LDS_BYTES);
+  if (by_lds < per_cu) per_cu = by_lds;
+  const int64_t grid = persistent_grid(frames * tiles, per_cu);
+  k_base_step_cols<N, E, C><<<dim3((unsigned)grid), dim3(Sh::T * C), 0, s>>>(
+      tb, measured, order, count, amp, loss_part, width, tiles, frames * tiles, gscale);
+}
+
 template <int N, int E>
 struct StepColsRun {
   static ptx_status run(float2* tb, const float* measured, const int64_t* order, int64_t count, float* amp,
                         float* loss_part, int64_t frames, int64_t width, float gscale, hipStream_t s) {
     using Cf = ColCfg<N, E>;
-    if (width >= Cf::CMAX) {
-      constexpr int C = Cf::CMAX;
-      const int64_t tiles = width / C;
-      const int64_t grid = persistent_grid(frames * tiles, 1024 / (Cf::T * C));
-      k_base_step_cols<N, E, C><<<dim3((unsigned)grid), dim3(Cf::T * C), 0, s>>>(
-          tb, measured, order, count, amp, loss_part, width, tiles, frames * tiles, gscale);
-    } else {
-      constexpr int C = 8;
-      const int64_t tiles = width / C;
-      const int64_t grid = persistent_grid(frames * tiles, 1024 / (Cf::T * C));
-      k_base_step_cols<N, E, C><<<dim3((unsigned)grid), dim3(Cf::T * C), 0, s>>>(
-          tb, measured, order, count, amp, loss_part, width, tiles, frames * tiles, gscale);
+    if (frames * (width / 4) >= (int64_t(1) << 31)) {  // the kernel counts tiles in 32 bits
+      set_error("base step cols: %lld frames of width %lld exceed the tile count limit", (long long)frames,
+                (long long)width);
+      return PTX_ERR_UNSUPPORTED;
     }
+    if constexpr (Cf::CMAX > 8) {
+      if (width < Cf::CMAX) {
+        launch_step_cols<N, E, 8>(tb, measured, order, count, amp, loss_part, frames, width, gscale, s);
+        return check_launch("base step cols", s);
+      }
+    }
+    launch_step_cols<N, E, Cf::CMAX>(tb, measured, order, count, amp, loss_part, frames, width, gscale, s);
     return check_launch("base step cols", s);
   }
 };
@@ -279,10 +344,10 @@ size_t ptx_base_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64
   if (check_shapes("ptx_base_step_workspace", H, W, h, w, B) != PTX_OK) return 0;
   ScatterPlan sp{};
   dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
-  int64_t tiles = 0;
-  dispatch_ne<ColTilesQuery>(PASS_COL, h, w, &tiles);
+  int64_t loss_parts = 0;
+  dispatch_ne<StepLossPartsQuery>(PASS_COL, h, w, &loss_parts);
   // room for the indexed variant's gathered positions too
-  return carve_scan(nullptr, H, W, h, w, B, flags, true, sp, tiles, true).bytes;
+  return carve_scan(nullptr, H, W, h, w, B, flags, true, sp, loss_parts, true).bytes;
 }
 
 }  // extern "C"
@@ -316,9 +381,9 @@ ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t
   }
   ScatterPlan sp{};
   dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
-  int64_t tiles = 0;
-  dispatch_ne<ColTilesQuery>(PASS_COL, h, w, &tiles);
-  const ScanWorkspace ws = carve_scan(workspace, H, W, h, w, B, flags, true, sp, tiles, order != nullptr);
+  int64_t loss_parts = 0;
+  dispatch_ne<StepLossPartsQuery>(PASS_COL, h, w, &loss_parts);
+  const ScanWorkspace ws = carve_scan(workspace, H, W, h, w, B, flags, true, sp, loss_parts, order != nullptr);
   if (!workspace || ws_bytes < ws.bytes) {
     set_error("%s: workspace of %zu bytes needed, %zu given", fn, ws.bytes, ws_bytes);
     return PTX_ERR_WORKSPACE;
@@ -336,7 +401,7 @@ ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t
   st = dispatch_ne<StepColsRun>(PASS_COL, h, ws.tb, measured, order, count, amplitude, ws.loss_part, B, w,
                                 2.0f * loss_scale, s);
   if (st != PTX_OK) return st;
-  loss_rows(ws.loss_part, tiles, B, 1.0f / (float)(h * w), loss_per_position, s);
+  loss_rows(ws.loss_part, loss_parts, B, 1.0f / (float)(h * w), loss_per_position, s);
   if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
   return scan_scatter<true>(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, bpos, B,
                             (float2*)grad_sample, (float2*)grad_probe, flags, 1.0f, s);

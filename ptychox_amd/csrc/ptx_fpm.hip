@@ -129,8 +129,9 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
   const int64_t blk = xcd_paired_block(blockIdx.x, gridDim.x);
-  const int64_t frame = blk / tiles_per_frame;
-  const int64_t tile = blk % tiles_per_frame;
+  const TileIdx ti = split_tile(blk, tiles_per_frame);
+  const int64_t frame = ti.frame;
+  const int64_t tile = ti.tile;
   const int64_t col = tile * C + c;
   const int64_t off = frame * N * width;  // frame base (uniform)
   float2 v[E];

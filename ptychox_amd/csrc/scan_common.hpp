@@ -114,14 +114,6 @@ struct GatherRowsRun {
     return check_launch(SIGN ? "base gather rows" : "scan gather rows", s);
   }
 };
-template <int N, int E>
-struct ColTilesQuery {
-  static ptx_status run(int64_t width, int64_t* tiles) {
-    using Cf = ColCfg<N, E>;
-    *tiles = width >= Cf::CMAX ? width / Cf::CMAX : width / 8;
-    return PTX_OK;
-  }
-};
 
 struct ScatterPlan {
   int S;        // rows per work-group
@@ -162,7 +154,7 @@ struct ScatterRowsRun {
 struct ScanWorkspace {
   float2* tb;        // [B, h, w]
   float2* gp_part;   // [nchunks, h, w]
-  float* loss_part;  // [B, col tiles]   (step only)
+  float* loss_part;  // [B, loss partials per frame]   (step only)
   GatherScratch gather;  // deterministic only
   int64_t* pos_batch;  // [B, 2] gathered positions (indexed step only)
   int* bad_index;      // set when an order entry is out of range (indexed step only)
@@ -170,12 +162,12 @@ struct ScanWorkspace {
 };
 
 ScanWorkspace carve_scan(void* base, int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags,
-                         bool step, const ScatterPlan& sp, int64_t col_tiles, bool indexed = false) {
+                         bool step, const ScatterPlan& sp, int64_t loss_parts, bool indexed = false) {
   Carver cv{(char*)base, 0};
   ScanWorkspace ws{};
   ws.tb = cv.take<float2>((size_t)(B * h * w));
   ws.gp_part = cv.take<float2>((size_t)(sp.nchunks * h * w));
-  ws.loss_part = step ? cv.take<float>((size_t)(B * col_tiles)) : nullptr;
+  ws.loss_part = step ? cv.take<float>((size_t)(B * loss_parts)) : nullptr;
   if (use_segments(flags)) ws.gather = carve_gather(cv, H, W, h, w, B);
   if (indexed) {
     ws.pos_batch = cv.take<int64_t>((size_t)(2 * B));
