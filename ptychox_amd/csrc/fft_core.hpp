@@ -225,15 +225,22 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restr
 #pragma unroll
       for (int m = 0; m < R; ++m) v[i + m * NB] = a[m];
     } else {
-      const int base = (j - k) * R + k;
+      // pad(base + m NS) = pad(base) + pad(m NS): the R outputs lie in one
+      // NS R-aligned block (base = (j - k) R + k, k < NS), which either sits
+      // inside one 16-entry pad group (NS R <= 16) or steps m NS by whole
+      // groups from an offset below 16 -- so one address per butterfly and
+      // compile-time offsets (checked exhaustively for N <= 4096, R <= 16)
+      float2* wb = lds + lds_pad((j - k) * R + k);
 #pragma unroll
-      for (int m = 0; m < R; ++m) lds[lds_pad(base + m * NS)] = a[m];
+      for (int m = 0; m < R; ++m) wb[lds_pad(m * NS)] = a[m];
     }
   }
   if constexpr (!LAST) {
     lds_barrier();
+    // pad(t + q T) = pad(t) + pad(q T) likewise (t < T, T a power of two)
+    const float2* rb = lds + lds_pad(t);
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = lds[lds_pad(t + q * T)];
+    for (int q = 0; q < E; ++q) v[q] = rb[lds_pad(q * T)];
   }
 }
 
