@@ -16,7 +16,9 @@
 // where v[q] is element t + q*(N/E) of the sequence.
 #pragma once
 
+#include "buffer_io.hpp"
 #include "fft_core.hpp"
+#include "ptx_internal.hpp"
 
 namespace ptx {
 
@@ -137,54 +139,130 @@ inline void launch_cols(const Op& op, int64_t frames, int64_t width, hipStream_t
 // rows already (inverse-)transformed, this completes  F^-1 (D . F x)  up to the
 // final row pass.  ``transfer`` is one [N, width] frame shared by every frame of
 // the batch (CONJ multiplies by its conjugate, for the adjoint).
+//
+// Column-tile-major and software-pipelined: work-group w owns one column tile
+// and a residue class of frames (f = group, group + ngroups, ...), so the
+// tile's transfer values are loaded once and stay in registers -- no transfer
+// traffic per frame (the 8 MB transfer of a 1024^2 frame does not fit an
+// XCD's L2).  Two register sets alternate, one holding the frame being
+// transformed and the other the loads of the next, in flight meanwhile.  A
+// frame past the end gets a zero-length buffer (its loads return zero without
+// touching memory), so every path through the loop issues the same memory
+// instructions and the compiler's wait counts stay exact; the first frame is
+// peeled for the same reason.  The exchanges alternate between two LDS
+// regions with one barrier each; the inverse transform starts on the region
+// after the forward's last, so consecutive exchanges never share a region and
+// no other barrier is needed.
+template <int N, int E, int C>
+struct PipeColsShape {
+  static constexpr int T = ColCfg<N, E>::T;
+  static constexpr int THREADS = T * C;
+  static constexpr int EXCHANGES = FftShape<N, E>::STAGES - 1;
+  static constexpr size_t LDS_BYTES = (2 * (size_t)C * col_stride<N, C>() + N) * sizeof(float2);
+  static_assert(LDS_BYTES <= 160 * 1024, "LDS budget");
+  // resident work-groups per CU: threads and LDS
+  static constexpr int PER_CU = (1024 / THREADS) < (int)((160 * 1024) / LDS_BYTES) ? (1024 / THREADS)
+                                                                                  : (int)((160 * 1024) / LDS_BYTES);
+};
+
 template <int N, int E, int C, bool CONJ>
 __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     k_transfer_cols(float2* __restrict__ tb, const float2* __restrict__ transfer, int64_t width,
-                    int64_t tiles_per_frame) {
-  using Cf = ColCfg<N, E>;
-  constexpr int T = Cf::T;
-  __shared__ float2 lds[C * col_stride<N, C>()];
+                    int64_t tiles_per_frame, int64_t frames, int64_t ngroups) {
+  using Sh = PipeColsShape<N, E, C>;
+  constexpr int T = Sh::T;
+  __shared__ float2 lds0[C * col_stride<N, C>()];
+  __shared__ float2 lds1[C * col_stride<N, C>()];
+  __shared__ float2 s_tw[N];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
-  const int64_t blk = xcd_paired_block(blockIdx.x, gridDim.x);
-  const TileIdx ti = split_tile(blk, tiles_per_frame);
-  const int64_t frame = ti.frame;
-  const int64_t col = ti.tile * C + c;
-  float2* p = tb + frame * N * width;
-  float2 v[E];
+  const int64_t fsz = (int64_t)N * width;               // elements per frame
+  const uint32_t fbytes = (uint32_t)(fsz * 8);
+  const uint32_t lane = (uint32_t)(t * (int)width + c);  // element offset inside a tile (row t, column c)
+  const uint32_t rstep = (uint32_t)(T * width);          // elements from row t + q T to t + (q + 1) T
+  const unsigned nf = (unsigned)frames, ng = (unsigned)ngroups;
+  // adjacent column tiles on one XCD (they share 128-byte lines through its L2)
+  const TileIdx ti = split_tile(xcd_paired_block(blockIdx.x, gridDim.x), tiles_per_frame);
+  const uint32_t col0 = (uint32_t)(ti.tile * C);
+  const unsigned group = (unsigned)ti.frame;  // < ngroups <= frames
+  load_lds_twiddles<N>(s_tw);
+  float2 tr[E];
+  {
+    const auto rtr = buffer_rsrc(transfer, fbytes);
 #pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = p[(t + q * T) * (int)width + (int)col];
-  fft_seq<N, E, -1>(v, t, &lds[c * col_stride<N, C>()]);
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const float2 tr = transfer[(t + q * T) * (int)width + (int)col];
-    v[q] = CONJ ? c_mulc(v[q], tr) : c_mul(v[q], tr);
+    for (int q = 0; q < E; ++q) tr[q] = buf_load_f2(rtr, lane * 8u, (q * rstep + col0) * 8u);
   }
-  lds_barrier();  // second transform reuses the LDS region
-  fft_seq<N, E, +1>(v, t, &lds[c * col_stride<N, C>()]);
+  auto fetch = [&](unsigned f, float2 (&dv)[E]) {
+    const bool live = f < nf;
+    const auto rt = buffer_rsrc(tb + (int64_t)(live ? f : 0u) * fsz, live ? fbytes : 0u);
 #pragma unroll
-  for (int q = 0; q < E; ++q) p[(t + q * T) * (int)width + (int)col] = v[q];
+    for (int q = 0; q < E; ++q) dv[q] = buf_load_f2(rt, lane * 8u, (q * rstep + col0) * 8u);
+  };
+  auto process = [&](unsigned f, float2 (&v)[E]) {
+    // the exchange addresses depend only on (t, c); recompute them per frame
+    // from the thread index (one register kept live across the loop) instead
+    // of letting them be hoisted out of the loop and held in registers
+    unsigned tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int tt = (int)(tid / C), cc = (int)(tid % C);
+    float2* l0 = &lds0[cc * col_stride<N, C>()];
+    float2* l1 = &lds1[cc * col_stride<N, C>()];
+    fft_stages<N, E, -1, true, 0, true>(v, tt, l0, l1, s_tw);
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = CONJ ? c_mulc(v[q], tr[q]) : c_mul(v[q], tr[q]);
+    if constexpr (Sh::EXCHANGES % 2 == 1)
+      fft_stages<N, E, +1, true, 0, true>(v, tt, l1, l0, s_tw);
+    else
+      fft_stages<N, E, +1, true, 0, true>(v, tt, l0, l1, s_tw);
+    const auto rt = buffer_rsrc(tb + (int64_t)f * fsz, fbytes);
+#pragma unroll
+    for (int q = 0; q < E; ++q) buf_store_f2(v[q], rt, lane * 8u, (q * rstep + col0) * 8u);
+  };
+  float2 va[E], vb[E];
+  unsigned f = group;
+  fetch(f, va);
+  fetch(f + ng, vb);  // in flight during this frame's transforms
+  process(f, va);
+  f += ng;
+  while (f < nf) {
+    fetch(f + ng, va);
+    process(f, vb);
+    f += ng;
+    if (f >= nf) break;
+    fetch(f + ng, vb);
+    process(f, va);
+    f += ng;
+  }
 }
 
-// host launcher (check_launch is the caller's)
+// host launcher (check_launch is the caller's): tiles_per_frame x ngroups
+// work-groups, ngroups = the frame groups that fill the resident slots
+template <int N, int E, int C>
+inline void launch_transfer_cols_c(float2* tb, const float2* transfer, int64_t frames, int64_t width, bool conj,
+                                   hipStream_t s) {
+  using Sh = PipeColsShape<N, E, C>;
+  const int64_t tiles = width / C;
+  int64_t ngroups = (int64_t)device_cu_count() * Sh::PER_CU / tiles;
+  if (ngroups > frames) ngroups = frames;
+  if (ngroups < 1) ngroups = 1;
+  const dim3 grid((unsigned)(tiles * ngroups));
+  if (conj)
+    k_transfer_cols<N, E, C, true><<<grid, dim3(Sh::THREADS), 0, s>>>(tb, transfer, width, tiles, frames, ngroups);
+  else
+    k_transfer_cols<N, E, C, false><<<grid, dim3(Sh::THREADS), 0, s>>>(tb, transfer, width, tiles, frames, ngroups);
+}
+
 template <int N, int E>
 inline void launch_transfer_cols(float2* tb, const float2* transfer, int64_t frames, int64_t width, bool conj,
                                  hipStream_t s) {
   using Cf = ColCfg<N, E>;
-  const int C = width >= Cf::CMAX ? Cf::CMAX : 8;
-  const int64_t tiles = width / C;
-  const dim3 grid((unsigned)(frames * tiles));
-  if (C == Cf::CMAX) {
-    if (conj)
-      k_transfer_cols<N, E, Cf::CMAX, true><<<grid, dim3(Cf::T * Cf::CMAX), 0, s>>>(tb, transfer, width, tiles);
-    else
-      k_transfer_cols<N, E, Cf::CMAX, false><<<grid, dim3(Cf::T * Cf::CMAX), 0, s>>>(tb, transfer, width, tiles);
-  } else {
-    if (conj)
-      k_transfer_cols<N, E, 8, true><<<grid, dim3(Cf::T * 8), 0, s>>>(tb, transfer, width, tiles);
-    else
-      k_transfer_cols<N, E, 8, false><<<grid, dim3(Cf::T * 8), 0, s>>>(tb, transfer, width, tiles);
+  if constexpr (Cf::CMAX > 8) {
+    if (width < Cf::CMAX) {  // narrow frames: 8 columns per tile
+      launch_transfer_cols_c<N, E, 8>(tb, transfer, frames, width, conj, s);
+      return;
+    }
   }
+  launch_transfer_cols_c<N, E, Cf::CMAX>(tb, transfer, frames, width, conj, s);
 }
 
 // ---------------------------------------------------------------- plain ops
