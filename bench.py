@@ -77,9 +77,11 @@ def kernel_bytes(label: str, B: int, N: int, H: int, W: int, nparts: int, atomic
         # read 8N + re-read sample window 8N per position, then either the object
         # read-modify-write 16N (atomic) or the window contribution write 8N
         # (sorted segments); probe read 8N and probe partials nparts*8N once
+        # (the tile lists and per-view loss sums ride along: < 0.2 MB)
         "base scatter rows": B * (32 if atomic else 24) * N + 8 * N + nparts * 8 * N,
-        # every window contribution read once (8N per position) + object written once
-        "tile gather": B * 8 * N + 8 * H * W,
+        # every window contribution read once (8N per position) + object written
+        # once; the probe-partial sum (nparts*8N read, 8N written) rides along
+        "tile gather": B * 8 * N + 8 * H * W + nparts * 8 * N + 8 * N,
         "probe partial sum": nparts * 8 * N + 8 * N,
         "zero fill": 8 * H * W,
     }

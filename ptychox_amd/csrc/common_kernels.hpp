@@ -24,36 +24,43 @@ inline void fill_zero2(float2* p, int64_t n, hipStream_t s) {
 }
 
 // out[i] (+)= sum_{c < nparts} part[c*n + i], summed in ascending c (deterministic)
-// conj: out (+)= conj(sum of the parts)
-static __global__ void __launch_bounds__(256) k_sum_parts2(const float2* __restrict__ part, int64_t nparts, int64_t n,
-                                                    float2* __restrict__ out, int accumulate, int conj) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  // eight slices' loads issued before their additions, which stay in ascending c
-  auto sum_into = [&](float2 s) {
-    int64_t c = 0;
-    for (; c + 8 <= nparts; c += 8) {
-      float2 v[8];
+// conj: out (+)= conj(sum of the parts).  One 256-thread work-group per 256 outputs.
+struct SumParts2 {
+  const float2* part;
+  int64_t nparts, n;
+  float2* out;
+  int accumulate, conj;
+  __host__ __device__ int64_t blocks() const { return n > 0 ? (n + 255) / 256 : 0; }
+  __device__ __forceinline__ void run(int64_t blk) const {
+    const int64_t i = blk * 256 + threadIdx.x;
+    if (i >= n) return;
+    // eight slices' loads issued before their additions, which stay in ascending c
+    auto sum_into = [&](float2 s) {
+      int64_t c = 0;
+      for (; c + 8 <= nparts; c += 8) {
+        float2 v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = part[(c + u) * n + i];
+        for (int u = 0; u < 8; ++u) v[u] = part[(c + u) * n + i];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) s = c_add(s, v[u]);
+        for (int u = 0; u < 8; ++u) s = c_add(s, v[u]);
+      }
+      for (; c < nparts; ++c) s = c_add(s, part[c * n + i]);
+      return s;
+    };
+    if (!conj) {
+      out[i] = sum_into(accumulate ? out[i] : c_zero());
+    } else {
+      float2 t = sum_into(c_zero());
+      t.y = -t.y;
+      out[i] = accumulate ? c_add(out[i], t) : t;
     }
-    for (; c < nparts; ++c) s = c_add(s, part[c * n + i]);
-    return s;
-  };
-  if (!conj) {
-    out[i] = sum_into(accumulate ? out[i] : c_zero());
-  } else {
-    float2 t = sum_into(c_zero());
-    t.y = -t.y;
-    out[i] = accumulate ? c_add(out[i], t) : t;
   }
-}
+};
+static __global__ void __launch_bounds__(256) k_sum_parts2(SumParts2 w) { w.run(blockIdx.x); }
 inline void sum_parts2(const float2* part, int64_t nparts, int64_t n, float2* out, int accumulate, hipStream_t s,
                        int conj = 0) {
-  if (n > 0)
-    k_sum_parts2<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(part, nparts, n, out, accumulate, conj);
+  const SumParts2 w{part, nparts, n, out, accumulate, conj};
+  if (w.blocks() > 0) k_sum_parts2<<<dim3((unsigned)w.blocks()), dim3(256), 0, s>>>(w);
 }
 
 
@@ -77,17 +84,27 @@ __device__ __forceinline__ float block_sum(float x, float* red /* >= 16 floats o
 // out[b] = scale * sum_{k < nparts} part[b*nparts + k]: one work-group per b,
 // each thread summing parts k = thread, thread + 256, ... in ascending order,
 // then the fixed-order work-group sum (deterministic)
-static __global__ void __launch_bounds__(256) k_loss_rows(const float* __restrict__ part, int64_t nparts,
-                                                   float scale, float* __restrict__ out) {
+struct LossRows {
+  const float* part;
+  int64_t nparts;
+  float scale;
+  float* out;
+  int64_t batch;  // 0: nothing to do
+  __host__ __device__ int64_t blocks() const { return batch; }
+  __device__ __forceinline__ void run(int64_t b, float* red /* 16 floats of LDS */) const {
+    float s = 0.0f;
+    for (int64_t k = threadIdx.x; k < nparts; k += 256) s += part[b * nparts + k];
+    const float tot = block_sum(s, red);
+    if (threadIdx.x == 0) out[b] = tot * scale;
+  }
+};
+static __global__ void __launch_bounds__(256) k_loss_rows(LossRows w) {
   __shared__ float red[16];
-  const int64_t b = blockIdx.x;
-  float s = 0.0f;
-  for (int64_t k = threadIdx.x; k < nparts; k += 256) s += part[b * nparts + k];
-  const float tot = block_sum(s, red);
-  if (threadIdx.x == 0) out[b] = tot * scale;
+  w.run(blockIdx.x, red);
 }
 inline void loss_rows(const float* part, int64_t nparts, int64_t batch, float scale, float* out, hipStream_t s) {
-  if (batch > 0) k_loss_rows<<<dim3((unsigned)batch), dim3(256), 0, s>>>(part, nparts, scale, out);
+  const LossRows w{part, nparts, scale, out, batch};
+  if (w.blocks() > 0) k_loss_rows<<<dim3((unsigned)w.blocks()), dim3(256), 0, s>>>(w);
 }
 
 // ---------------------------------------------------------------- sorted-segment scatter
@@ -107,38 +124,77 @@ __device__ __forceinline__ int64_t pmod(int64_t a, int64_t n) {
 // ptychography sub-spectrum of python/ptychoX/fpm/ops.py:113-124.
 
 // For every grid tile, the views whose window overlaps it, in ascending view
-// order.  lists: [ntiles, batch] int32; counts: [ntiles].
-static __global__ void __launch_bounds__(256) k_tile_lists(const int64_t* __restrict__ pos, int64_t batch, int h, int w,
-                                                           int64_t H, int64_t W, int64_t off_y, int64_t off_x,
-                                                           int64_t tiles_x, int* __restrict__ lists,
-                                                           int* __restrict__ counts) {
-  __shared__ int wave_tot[4];
-  const int64_t tile = blockIdx.x;
-  const int64_t ty0 = (tile / tiles_x) * kTileH, tx0 = (tile % tiles_x) * kTileW;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int count = 0;
-  for (int64_t base = 0; base < batch; base += 256) {
-    const int64_t b = base + threadIdx.x;
-    bool hit = false;
-    if (b < batch) {
-      const int64_t oy = pmod(pos[2 * b + 1] + off_y, H), ox = pmod(pos[2 * b] + off_x, W);
-      // two arcs on a circle overlap iff one starts inside the other
-      const bool ry = pmod(ty0 - oy, H) < h || pmod(oy - ty0, H) < kTileH;
-      const bool rx = pmod(tx0 - ox, W) < w || pmod(ox - tx0, W) < kTileW;
-      hit = ry && rx;
+// order.  lists: [ntiles, batch] int32; counts: [ntiles].  One 256-thread
+// work-group per tile.
+struct TileLists {
+  const int64_t* pos;
+  int64_t batch;
+  int h, w;
+  int64_t H, W, off_y, off_x, tiles_x;
+  int* lists;
+  int* counts;
+  int64_t ntiles;  // 0: nothing to do
+  __host__ __device__ int64_t blocks() const { return ntiles; }
+  __device__ __forceinline__ void run(int64_t tile, int* wave_tot /* 4 ints of LDS */) const {
+    const int64_t ty0 = (tile / tiles_x) * kTileH, tx0 = (tile % tiles_x) * kTileW;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int count = 0;
+    for (int64_t base = 0; base < batch; base += 256) {
+      const int64_t b = base + threadIdx.x;
+      bool hit = false;
+      if (b < batch) {
+        const int64_t oy = pmod(pos[2 * b + 1] + off_y, H), ox = pmod(pos[2 * b] + off_x, W);
+        // two arcs on a circle overlap iff one starts inside the other
+        const bool ry = pmod(ty0 - oy, H) < h || pmod(oy - ty0, H) < kTileH;
+        const bool rx = pmod(tx0 - ox, W) < w || pmod(ox - tx0, W) < kTileW;
+        hit = ry && rx;
+      }
+      const unsigned long long m = __ballot(hit);
+      const int before = __popcll(m & ((1ull << lane) - 1ull));
+      if (lane == 0) wave_tot[wave] = __popcll(m);
+      __syncthreads();
+      int off = count;
+      for (int k = 0; k < wave; ++k) off += wave_tot[k];
+      if (hit) lists[tile * batch + off + before] = (int)b;
+      count += wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+      __syncthreads();
     }
-    const unsigned long long m = __ballot(hit);
-    const int before = __popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) wave_tot[wave] = __popcll(m);
-    __syncthreads();
-    int off = count;
-    for (int k = 0; k < wave; ++k) off += wave_tot[k];
-    if (hit) lists[tile * batch + off + before] = (int)b;
-    count += wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
-    __syncthreads();
+    if (threadIdx.x == 0) counts[tile] = count;
   }
-  if (threadIdx.x == 0) counts[tile] = count;
+};
+static __global__ void __launch_bounds__(256) k_tile_lists(TileLists w) {
+  __shared__ int wave_tot[4];
+  w.run(blockIdx.x, wave_tot);
 }
+
+// Independent small jobs that ride along as extra work-groups of a larger
+// 256-thread launch (the work-groups after the launch's own): per-tile view
+// lists, per-view loss sums, probe-partial sums.  Each work-group does exactly
+// what the stand-alone kernel's work-group would, so results are unchanged;
+// the launches disappear (neutral on the conventional step, +3 % on the
+// near-field step).  Placed before the launch's own work-groups they measured
+// slower: they hold the big kernel's LDS allocation and delay its work.
+struct SideWork {
+  TileLists lists{};
+  LossRows loss{};
+  SumParts2 sums{};
+  __host__ __device__ int64_t blocks() const { return lists.blocks() + loss.blocks() + sums.blocks(); }
+  __device__ __forceinline__ void run(int64_t blk) const {
+    __shared__ int wave_tot[4];
+    __shared__ float red[16];
+    if (blk < lists.blocks()) {
+      lists.run(blk, wave_tot);
+      return;
+    }
+    blk -= lists.blocks();
+    if (blk < loss.blocks()) {
+      loss.run(blk, red);
+      return;
+    }
+    blk -= loss.blocks();
+    sums.run(blk);
+  }
+};
 
 // grad[Y, X] (+)= sum over listed views b (ascending) of contrib[b, cy, cx] with
 // (cy, cx) = ((Y - oy_b) mod H, (X - ox_b) mod W) inside the h x w window.
@@ -225,22 +281,27 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
                                                             const int* __restrict__ n_items,
                                                             float2* __restrict__ partial,
                                                             float2* __restrict__ grad, int64_t H, int64_t W,
-                                                            int accumulate) {
+                                                            int accumulate, SideWork side, int64_t main_blocks) {
   constexpr int RPT = kTilePx / 256;  // rows per thread
   constexpr int RSTEP = 256 / kTileW;
   constexpr int CHUNK = SEG ? kSegViews : kListChunk;
+  if ((int64_t)blockIdx.x >= main_blocks) {  // work-groups riding along
+    side.run((int64_t)blockIdx.x - main_blocks);
+    return;
+  }
+  const unsigned blk = blockIdx.x;
   __shared__ int s_b[CHUNK];
   __shared__ int s_oy[CHUNK];
   __shared__ int s_ox[CHUNK];
   int64_t tile;
   int seg = 0;
   if (SEG) {
-    const int item = blockIdx.x;
+    const int item = (int)blk;
     if (item >= *n_items) return;  // the grid is the host's upper bound
     tile = item_tile[item];
     seg = item - seg_off[tile];
   } else {
-    tile = blockIdx.x;
+    tile = blk;
   }
   const int64_t ty0 = (tile / tiles_x) * kTileH, tx0 = (tile % tiles_x) * kTileW;
   const int64_t X = tx0 + (threadIdx.x % kTileW);
@@ -402,26 +463,39 @@ inline GatherScratch carve_gather(Carver& cv, int64_t H, int64_t W, int64_t h, i
 // grad (+)= the windows' contributions (contrib [B, h, w], window corners
 // pos + (off_x, off_y), wrapping modulo (H, W)): tile lists, segment plan,
 // segmented gather, combine.
+// The list job of a gather (to run as side work of an earlier launch).
+inline TileLists gather_lists_work(const GatherScratch& g, const int64_t* pos, int64_t B, int64_t h, int64_t w,
+                                   int64_t H, int64_t W, int64_t off_y, int64_t off_x) {
+  const TileGrid tg = tile_grid(H, W);
+  return TileLists{pos, B, (int)h, (int)w, H, W, off_y, off_x, tg.tiles_x, g.lists, g.counts, tg.count()};
+}
+
+// grad (+)= the windows' contributions: tile lists (unless lists_done: an
+// earlier launch ran gather_lists_work), then the single-pass gather or plan,
+// segmented gather and combine.  `side` rides along with the gather launch.
 inline ptx_status run_tile_gather(const GatherScratch& g, const float2* contrib, int64_t h, int64_t w,
                                   const int64_t* pos, int64_t B, int64_t off_y, int64_t off_x, float2* grad,
-                                  int64_t H, int64_t W, int accumulate, hipStream_t s) {
+                                  int64_t H, int64_t W, int accumulate, hipStream_t s, bool lists_done = false,
+                                  const SideWork& side = SideWork{}) {
   const TileGrid tg = tile_grid(H, W);
   ptx_status st;
-  k_tile_lists<<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(pos, B, (int)h, (int)w, H, W, off_y, off_x,
-                                                                 tg.tiles_x, g.lists, g.counts);
-  if ((st = check_launch("tile lists", s)) != PTX_OK) return st;
+  if (!lists_done) {
+    const TileLists tl = gather_lists_work(g, pos, B, h, w, H, W, off_y, off_x);
+    k_tile_lists<<<dim3((unsigned)tl.blocks()), dim3(256), 0, s>>>(tl);
+    if ((st = check_launch("tile lists", s)) != PTX_OK) return st;
+  }
   if (!gather_segmented(h, w, B, H, W)) {
-    k_tile_gather<false><<<dim3((unsigned)tg.count()), dim3(256), 0, s>>>(
+    k_tile_gather<false><<<dim3((unsigned)(tg.count() + side.blocks())), dim3(256), 0, s>>>(
         contrib, (int)h, (int)w, pos, B, off_y, off_x, g.lists, g.counts, tg.tiles_x, nullptr, nullptr, nullptr,
-        nullptr, nullptr, grad, H, W, accumulate);
+        nullptr, nullptr, grad, H, W, accumulate, side, tg.count());
     return check_launch("tile gather", s);
   }
   const GatherBounds gb = gather_bounds(tg.count(), h, w, B);
   k_gather_plan<<<dim3(1), dim3(1024), 0, s>>>(g.counts, tg.count(), g.seg_off, g.slot_off, g.item_tile, g.n_items);
   if ((st = check_launch("gather plan", s)) != PTX_OK) return st;
-  k_tile_gather<true><<<dim3((unsigned)gb.items), dim3(256), 0, s>>>(
+  k_tile_gather<true><<<dim3((unsigned)(gb.items + side.blocks())), dim3(256), 0, s>>>(
       contrib, (int)h, (int)w, pos, B, off_y, off_x, g.lists, g.counts, tg.tiles_x, g.seg_off, g.slot_off,
-      g.item_tile, g.n_items, g.partial, grad, H, W, accumulate);
+      g.item_tile, g.n_items, g.partial, grad, H, W, accumulate, side, gb.items);
   if ((st = check_launch("tile gather", s)) != PTX_OK) return st;
   if (gb.slots > 0) {
     k_gather_combine<<<dim3((unsigned)(tg.count() * (kTilePx / 256))), dim3(256), 0, s>>>(g.counts, g.slot_off,

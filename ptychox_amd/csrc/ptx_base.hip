@@ -401,10 +401,9 @@ ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t
   st = dispatch_ne<StepColsRun>(PASS_COL, h, ws.tb, measured, order, count, amplitude, ws.loss_part, B, w,
                                 2.0f * loss_scale, s);
   if (st != PTX_OK) return st;
-  loss_rows(ws.loss_part, loss_parts, B, 1.0f / (float)(h * w), loss_per_position, s);
-  if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
+  const LossRows loss{ws.loss_part, loss_parts, 1.0f / (float)(h * w), loss_per_position, B};
   return scan_scatter<true>(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, bpos, B,
-                            (float2*)grad_sample, (float2*)grad_probe, flags, 1.0f, s);
+                            (float2*)grad_sample, (float2*)grad_probe, flags, 1.0f, s, loss);
 }
 
 }  // namespace

@@ -273,11 +273,10 @@ ptx_status ptx_nearfield_step(const void* sample, int64_t H, int64_t W, const vo
   if ((st = dispatch_ne<StepRowsRun>(PASS_ROW, w, ws.tb, measured, amplitude, ws.loss_part, B * h, 2.0f * loss_scale, inv_n,
                                     s)) != PTX_OK)
     return st;
-  loss_rows(ws.loss_part, h, B, inv_n, loss_per_position, s);
-  if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
   if ((st = dispatch_ne<TransferColsRun>(PASS_COL, h, ws.tb, (const float2*)transfer, B, w, true, s)) != PTX_OK) return st;
+  const LossRows loss{ws.loss_part, h, inv_n, loss_per_position, B};
   return scan_scatter<false>(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, pos, B,
-                             (float2*)grad_sample, (float2*)grad_probe, flags, inv_n, s);
+                             (float2*)grad_sample, (float2*)grad_probe, flags, inv_n, s, loss);
 }
 
 }  // extern "C"

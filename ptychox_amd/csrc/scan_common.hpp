@@ -54,14 +54,19 @@ __global__ void __launch_bounds__(256)
     k_scan_scatter_rows(const float2* __restrict__ tb, const float2* __restrict__ sample, int64_t H, int64_t W,
                         const float2* __restrict__ probe, int h, const int64_t* __restrict__ pos, int64_t B,
                         int chunk, int ngroups, float2* __restrict__ gobj, float2* __restrict__ contrib,
-                        float2* __restrict__ gp_part, float scale) {
+                        float2* __restrict__ gp_part, float scale, SideWork side, int64_t main_blocks) {
   using Cf = RowCfg<N, E>;
   constexpr int T = Cf::T, S = Cf::S;
+  if ((int64_t)blockIdx.x >= main_blocks) {  // work-groups riding along
+    side.run((int64_t)blockIdx.x - main_blocks);
+    return;
+  }
+  const int blk = (int)blockIdx.x;
   __shared__ float2 lds[S * lds_len<N>()];
   const int seq = threadIdx.x / T;
   const int t = threadIdx.x % T;
-  const int group = blockIdx.x % ngroups;
-  const int ch = blockIdx.x / ngroups;
+  const int group = blk % ngroups;
+  const int ch = blk / ngroups;
   const int r = group * S + seq;
   const bool rv = r < h;
   const int64_t b0 = (int64_t)ch * chunk;
@@ -144,9 +149,11 @@ struct ScatterRowsRun {
   template <bool SIGN>
   static ptx_status run(const float2* tb, const float2* sample, int64_t H, int64_t W, const float2* probe, int h,
                         const int64_t* pos, int64_t B, const ScatterPlan& p, float2* gobj, float2* contrib,
-                        float2* gp_part, float scale, hipStream_t s, std::integral_constant<bool, SIGN>) {
-    k_scan_scatter_rows<N, E, SIGN><<<dim3((unsigned)(p.ngroups * p.nchunks)), dim3(256), 0, s>>>(
-        tb, sample, H, W, probe, h, pos, B, p.chunk, p.ngroups, gobj, contrib, gp_part, scale);
+                        float2* gp_part, float scale, const SideWork& side, hipStream_t s,
+                        std::integral_constant<bool, SIGN>) {
+    const int64_t main_blocks = (int64_t)p.ngroups * p.nchunks;
+    k_scan_scatter_rows<N, E, SIGN><<<dim3((unsigned)(main_blocks + side.blocks())), dim3(256), 0, s>>>(
+        tb, sample, H, W, probe, h, pos, B, p.chunk, p.ngroups, gobj, contrib, gp_part, scale, side, main_blocks);
     return check_launch(SIGN ? "base scatter rows" : "scan scatter rows", s);
   }
 };
@@ -195,11 +202,14 @@ ptx_status check_shapes(const char* fn, int64_t H, int64_t W, int64_t h, int64_t
   return PTX_OK;
 }
 
-// shared tail of backward and step: R2 scatter, probe reduction, deterministic gather
+// shared tail of backward and step: R2 scatter, probe reduction, deterministic
+// gather.  The tile lists (they need only the positions) and the step's
+// per-view loss sums (`loss`, batch 0 for none) ride along with the scatter
+// launch, the probe-partial sum with the gather launch.
 template <bool SIGN>
 ptx_status scan_scatter(const ScanWorkspace& ws, const ScatterPlan& sp, const float2* sample, int64_t H, int64_t W,
                         const float2* probe, int64_t h, int64_t w, const int64_t* pos, int64_t B, float2* gs,
-                        float2* gp, unsigned flags, float scale, hipStream_t s) {
+                        float2* gp, unsigned flags, float scale, hipStream_t s, const LossRows& loss = LossRows{}) {
   const bool det = use_segments(flags);
   const bool acc = flags & PTX_FLAG_ACCUMULATE;
   ptx_status st = PTX_OK;
@@ -207,15 +217,21 @@ ptx_status scan_scatter(const ScanWorkspace& ws, const ScatterPlan& sp, const fl
     fill_zero2(gs, H * W, s);
     if ((st = check_launch("zero fill", s)) != PTX_OK) return st;
   }
-  st = dispatch_ne<ScatterRowsRun>(PASS_ROW_LOOP, w, ws.tb, sample, H, W, probe, (int)h, pos, B, sp, gs, det ? ws.tb : nullptr,
-                                  ws.gp_part, scale, s, std::integral_constant<bool, SIGN>{});
+  SideWork with_scatter{};
+  if (det) with_scatter.lists = gather_lists_work(ws.gather, pos, B, h, w, H, W, 0, 0);
+  with_scatter.loss = loss;
+  st = dispatch_ne<ScatterRowsRun>(PASS_ROW_LOOP, w, ws.tb, sample, H, W, probe, (int)h, pos, B, sp, gs,
+                                  det ? ws.tb : nullptr, ws.gp_part, scale, with_scatter, s,
+                                  std::integral_constant<bool, SIGN>{});
   if (st != PTX_OK) return st;
-  sum_parts2(ws.gp_part, sp.nchunks, h * w, gp, acc ? 1 : 0, s);
-  if ((st = check_launch("probe partial sum", s)) != PTX_OK) return st;
-  if (det) {
-    if ((st = run_tile_gather(ws.gather, ws.tb, h, w, pos, B, 0, 0, gs, H, W, acc ? 1 : 0, s)) != PTX_OK) return st;
+  const SumParts2 probe_sum{ws.gp_part, sp.nchunks, h * w, gp, acc ? 1 : 0, 0};
+  if (!det) {
+    sum_parts2(ws.gp_part, sp.nchunks, h * w, gp, acc ? 1 : 0, s);
+    return check_launch("probe partial sum", s);
   }
-  return PTX_OK;
+  SideWork with_gather{};
+  with_gather.sums = probe_sum;
+  return run_tile_gather(ws.gather, ws.tb, h, w, pos, B, 0, 0, gs, H, W, acc ? 1 : 0, s, true, with_gather);
 }
 
 }  // namespace
