@@ -33,7 +33,10 @@ def _case(rng, n, B, step=8):
     return o, mask.astype(np.complex64), t.astype(np.complex64), t.astype(np.complex64), shifts
 
 
-@pytest.mark.parametrize("n,B,ds", [(16, 2, 2), (64, 9, 1), (128, 9, 2), (256, 4, 4)])
+# (8, 4, 2): 8-column frames (the narrow column tiles); 8x8 with 3 views and ds 1
+# is FP32-limited (the reference torch path's own coded-mask gradient is 1e-4 off the
+# FP64 oracle there), so that shape is not a parity case
+@pytest.mark.parametrize("n,B,ds", [(8, 4, 2), (16, 2, 2), (64, 9, 1), (128, 9, 2), (256, 4, 4)])
 def test_coded_vs_oracle(gpu, n, B, ds):
     from ptychox_amd import ptychoX_coded
 

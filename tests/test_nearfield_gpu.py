@@ -30,8 +30,8 @@ def _transfer(n, fresnel=20.0):
     return np.exp(1j * np.pi * lz * (f[:, None] ** 2 + f[None, :] ** 2)).astype(np.complex64)
 
 
-@pytest.mark.parametrize("H,h,B,det", [(64, 32, 4, False), (256, 128, 8, False), (256, 128, 8, True),
-                                       (400, 256, 5, False)])
+@pytest.mark.parametrize("H,h,B,det", [(16, 8, 3, False), (64, 32, 4, False), (256, 128, 8, False),
+                                       (256, 128, 8, True), (400, 256, 5, False)])
 def test_nearfield_vs_oracle(gpu, H, h, B, det):
     from ptychox_amd import ptychoX_nearfield, set_scatter_mode
 
