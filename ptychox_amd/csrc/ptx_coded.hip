@@ -337,57 +337,83 @@ __global__ void __launch_bounds__(256)
 
 // Q4: per (column tile, chunk of views): column FFT; spectrum partial
 // += conj(H1 R_b) v / N, with the view-independent conj(H1)/N applied once
-// after the view loop.  Views are pipelined as in k_coded_y_rows (next view's
-// tile and row-ramp values loaded during this view's transform).
+// after the view loop.  About one work-group per CU (ColChunkQuery), each on
+// one column tile and a chunk of consecutive views, software-pipelined like
+// k_transfer_cols: two register sets alternate, the next view's tile and ramp
+// values in flight during this view's transform; a view past the chunk gets
+// zero-length buffers (loads issued, no memory touched), so every path issues
+// the same memory instructions and the wait counts keep the prefetch in
+// flight; the first view is peeled; the exchanges alternate two LDS regions
+// (with an odd number of exchanges per transform, every other view starts on
+// the second region).
 template <int N, int E, int C>
 __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     k_coded_spec_cols(const float2* __restrict__ tb, const float2* __restrict__ h1, const float2* __restrict__ ex,
                       const float2* __restrict__ ey, int64_t B, int64_t width, int64_t tiles, int chunk, float inv_n,
                       float2* __restrict__ gs_part) {
-  using Cf = ColCfg<N, E>;
-  constexpr int T = Cf::T;
-  __shared__ float2 lds[C * col_stride<N, C>()];
+  using Sh = PipeColsShape<N, E, C>;
+  constexpr int T = Sh::T;
+  constexpr bool ODD = Sh::EXCHANGES % 2 == 1;
+  __shared__ float2 lds0[C * col_stride<N, C>()];
+  __shared__ float2 lds1[C * col_stride<N, C>()];
   __shared__ float2 s_tw[N];
   load_lds_twiddles<N>(s_tw);
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
-  const int64_t blk = xcd_paired_block(blockIdx.x, gridDim.x);
-  const int64_t tile = blk % tiles;
-  const int ch = (int)(blk / tiles);
+  const TileIdx ti = split_tile(xcd_paired_block(blockIdx.x, gridDim.x), tiles);
+  const int64_t tile = ti.tile;
+  const unsigned ch = (unsigned)ti.frame;  // chunk
   const int64_t col = tile * C + c;
-  const int64_t b0 = (int64_t)ch * chunk;
-  const int64_t b1 = b0 + chunk < B ? b0 + chunk : B;
+  const unsigned b0 = ch * (unsigned)chunk;                            // < B
+  const unsigned b1 = b0 + chunk < (unsigned)B ? b0 + chunk : (unsigned)B;
   const uint32_t fbytes = (uint32_t)(N * width * 8);
   const uint32_t lane = (uint32_t)((t * width + c) * 8);
   const uint32_t col0 = (uint32_t)(tile * C * 8);
   const uint32_t rstep = (uint32_t)(T * width * 8);
-  float2 acc[E], nv[E], ny[E], nrx;
+  float2 acc[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) acc[q] = c_zero();
-  auto fetch = [&](int64_t b, float2 (&dv)[E], float2 (&dy)[E], float2& drx) {
-    drx = ex[b * width + col];
+  auto fetch = [&](unsigned b, float2 (&dv)[E], float2 (&dy)[E], float2& drx) {
+    const bool live = b < b1;
+    const unsigned bb = live ? b : b0;
+    drx = buf_load_f2(buffer_rsrc(ex + (int64_t)bb * width, live ? (uint32_t)(width * 8) : 0u), (uint32_t)c * 8u,
+                      col0);
+    const auto ry = buffer_rsrc(ey + (int64_t)bb * N, live ? (uint32_t)(N * 8) : 0u);
 #pragma unroll
-    for (int q = 0; q < E; ++q) dy[q] = ey[b * N + t + q * T];
-    const auto r = buffer_rsrc(tb + b * N * width, fbytes);
+    for (int q = 0; q < E; ++q) dy[q] = buf_load_f2(ry, (uint32_t)t * 8u, (uint32_t)(q * T * 8));
+    const auto r = buffer_rsrc(tb + (int64_t)bb * N * width, live ? fbytes : 0u);
 #pragma unroll
     for (int q = 0; q < E; ++q) dv[q] = buf_load_f2(r, lane, q * rstep + col0);
   };
-  if (b0 < b1) fetch(b0, nv, ny, nrx);
-  for (int64_t b = b0; b < b1; ++b) {
-    float2 v[E], ry[E];
-#pragma unroll
-    for (int q = 0; q < E; ++q) {
-      v[q] = nv[q];
-      ry[q] = ny[q];
-    }
-    const float2 rx = nrx;
-    if (b + 1 < b1) fetch(b + 1, nv, ny, nrx);
-    int tt = t;
-    asm volatile("" : "+v"(tt));  // keep the exchange addresses per view, not hoisted into registers
-    fft_seq_tw<N, E, -1>(v, tt, &lds[c * col_stride<N, C>()], s_tw);
+  auto process = [&](float2 (&v)[E], const float2 (&ry)[E], float2 rx, bool second) {
+    // the exchange addresses depend only on (t, c): recomputed per view from the
+    // thread index instead of being hoisted out of the loop into registers
+    unsigned tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int tt = (int)(tid / C), cc = (int)(tid % C);
+    float2* l0 = &lds0[cc * col_stride<N, C>()];
+    float2* l1 = &lds1[cc * col_stride<N, C>()];
+    if (second)
+      fft_stages<N, E, -1, true, 0, true>(v, tt, l1, l0, s_tw);
+    else
+      fft_stages<N, E, -1, true, 0, true>(v, tt, l0, l1, s_tw);
 #pragma unroll
     for (int q = 0; q < E; ++q) acc[q] = c_add(acc[q], c_mulc(v[q], c_mul(ry[q], rx)));  // v conj(R_b)
-    lds_barrier();
+  };
+  float2 va[E], vb[E], ya[E], yb[E], xa, xb;
+  unsigned b = b0;
+  fetch(b, va, ya, xa);
+  fetch(b + 1, vb, yb, xb);  // in flight during this view's transform
+  process(va, ya, xa, false);
+  ++b;
+  while (b < b1) {
+    fetch(b + 1, va, ya, xa);
+    process(vb, yb, xb, ODD);
+    ++b;
+    if (b >= b1) break;
+    fetch(b + 1, vb, yb, xb);
+    process(va, ya, xa, false);
+    ++b;
   }
   float2* d = gs_part + (int64_t)ch * N * width;  // slice base; the lane adds its column
 #pragma unroll
@@ -463,12 +489,16 @@ struct RowChunkQuery {
     return PTX_OK;
   }
 };
+// Q4 chunks: about 256 work-groups (one per CU of the target, one resident
+// work-group each), so each walks a long chunk of views and few partial
+// spectra are written and summed.  A fixed target keeps the workspace size a
+// function of the shapes alone.
 template <int N, int E>
 struct ColChunkQuery {
   static ptx_status run(int64_t W, int64_t B, Chunks* c) {
     c->col_c = W >= ColCfg<N, E>::CMAX ? ColCfg<N, E>::CMAX : 8;
     c->col_tiles = (int)(W / c->col_c);
-    int64_t want = (2048 + c->col_tiles - 1) / c->col_tiles;
+    int64_t want = (256 + c->col_tiles - 1) / c->col_tiles;
     want = want < 1 ? 1 : (want > B ? B : want);
     c->col_chunk = (int)((B + want - 1) / want);
     c->col_nchunks = (int)((B + c->col_chunk - 1) / c->col_chunk);
@@ -489,14 +519,16 @@ struct SpecGradColsRun {
   static ptx_status run(const float2* tb, const float2* h1, const float2* ex, const float2* ey, int64_t B, int64_t W,
                         const Chunks& c, float inv_n, float2* gs_part, hipStream_t s) {
     const dim3 grid((unsigned)(c.col_tiles * c.col_nchunks));
-    if (c.col_c == ColCfg<N, E>::CMAX) {
-      constexpr int C = ColCfg<N, E>::CMAX;
-      k_coded_spec_cols<N, E, C><<<grid, dim3(ColCfg<N, E>::T * C), 0, s>>>(tb, h1, ex, ey, B, W, c.col_tiles, c.col_chunk,
-                                                                       inv_n, gs_part);
-    } else {
-      k_coded_spec_cols<N, E, 8><<<grid, dim3(ColCfg<N, E>::T * 8), 0, s>>>(tb, h1, ex, ey, B, W, c.col_tiles, c.col_chunk,
-                                                                       inv_n, gs_part);
+    constexpr int CM = ColCfg<N, E>::CMAX;
+    if constexpr (CM > 8) {
+      if (c.col_c != CM) {  // frames narrower than CMAX columns: tiles of 8
+        k_coded_spec_cols<N, E, 8><<<grid, dim3(ColCfg<N, E>::T * 8), 0, s>>>(tb, h1, ex, ey, B, W, c.col_tiles,
+                                                                         c.col_chunk, inv_n, gs_part);
+        return check_launch("coded spectrum-grad cols", s);
+      }
     }
+    k_coded_spec_cols<N, E, CM><<<grid, dim3(ColCfg<N, E>::T * CM), 0, s>>>(tb, h1, ex, ey, B, W, c.col_tiles,
+                                                                       c.col_chunk, inv_n, gs_part);
     return check_launch("coded spectrum-grad cols", s);
   }
 };
