@@ -423,12 +423,108 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   }
 }
 
+// P1 as a column-tile walker: a work-group owns one column tile and a chunk of
+// consecutive views; the tile's O^ H1 is loaded once, and per view only the
+// ramp factors are read (next view's in flight during this view's transform):
+// v = O^ H1 ey_b ex_b, column iFFT, store.  Zero-length buffers past the chunk,
+// the first view peeled and two-region LDS exchanges as in k_coded_spec_cols.
+template <int N, int E, int C>
+__global__ void __launch_bounds__((ColCfg<N, E>::T * C))
+    k_coded_spec_fwd_cols(const float2* __restrict__ spec, const float2* __restrict__ h1,
+                          const float2* __restrict__ ex, const float2* __restrict__ ey, int64_t B, int64_t width,
+                          int64_t tiles, int chunk, float2* __restrict__ out) {
+  using Sh = PipeColsShape<N, E, C>;
+  constexpr int T = Sh::T;
+  constexpr bool ODD = Sh::EXCHANGES % 2 == 1;
+  __shared__ float2 lds0[C * col_stride<N, C>()];
+  __shared__ float2 lds1[C * col_stride<N, C>()];
+  __shared__ float2 s_tw[N];
+  load_lds_twiddles<N>(s_tw);
+  const int c = threadIdx.x % C;
+  const int t = threadIdx.x / C;
+  const TileIdx ti = split_tile(xcd_paired_block(blockIdx.x, gridDim.x), tiles);
+  const int64_t col = ti.tile * C + c;
+  const unsigned b0 = (unsigned)ti.frame * (unsigned)chunk;  // < B
+  const unsigned b1 = b0 + chunk < (unsigned)B ? b0 + chunk : (unsigned)B;
+  const uint32_t fbytes = (uint32_t)(N * width * 8);
+  const uint32_t lane = (uint32_t)((t * width + c) * 8);
+  const uint32_t col0 = (uint32_t)(ti.tile * C * 8);
+  const uint32_t rstep = (uint32_t)(T * width * 8);
+  float2 base[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int64_t k = (int64_t)(t + q * T) * width + col;
+    base[q] = h1 ? c_mul(spec[k], h1[k]) : spec[k];  // h1 == null: H1 = 1
+  }
+  auto fetch = [&](unsigned b, float2 (&dy)[E], float2& drx) {
+    const bool live = b < b1;
+    const unsigned bb = live ? b : b0;
+    drx = buf_load_f2(buffer_rsrc(ex + (int64_t)bb * width, live ? (uint32_t)(width * 8) : 0u), (uint32_t)c * 8u,
+                      col0);
+    const auto ry = buffer_rsrc(ey + (int64_t)bb * N, live ? (uint32_t)(N * 8) : 0u);
+#pragma unroll
+    for (int q = 0; q < E; ++q) dy[q] = buf_load_f2(ry, (uint32_t)t * 8u, (uint32_t)(q * T * 8));
+  };
+  auto process = [&](unsigned b, const float2 (&ry)[E], float2 rx, bool second) {
+    unsigned tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int tt = (int)(tid / C), cc = (int)(tid % C);
+    float2* l0 = &lds0[cc * col_stride<N, C>()];
+    float2* l1 = &lds1[cc * col_stride<N, C>()];
+    float2 v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = c_mul(base[q], c_mul(ry[q], rx));
+    if (second)
+      fft_stages<N, E, +1, true, 0, true>(v, tt, l1, l0, s_tw);
+    else
+      fft_stages<N, E, +1, true, 0, true>(v, tt, l0, l1, s_tw);
+    const auto r = buffer_rsrc(out + (int64_t)b * N * width, fbytes);
+#pragma unroll
+    for (int q = 0; q < E; ++q) buf_store_f2(v[q], r, lane, q * rstep + col0);
+  };
+  float2 ya[E], yb[E], xa, xb;
+  unsigned b = b0;
+  fetch(b, ya, xa);
+  fetch(b + 1, yb, xb);
+  process(b, ya, xa, false);
+  ++b;
+  while (b < b1) {
+    fetch(b + 1, ya, xa);
+    process(b, yb, xb, ODD);
+    ++b;
+    if (b >= b1) break;
+    fetch(b + 1, yb, xb);
+    process(b, ya, xa, false);
+    ++b;
+  }
+}
+
 // ---------------------------------------------------------------- host launchers
 
+// P1: column tiles x chunks of views, about 256 work-groups
 template <int N, int E>
 struct SpecColsRun {
+  template <int C>
+  static void launch(const CodedSpecCols& op, int64_t frames, hipStream_t s) {
+    if (frames <= 0) return;
+    const int64_t tiles = op.width / C;
+    int64_t nchunks = (256 + tiles - 1) / tiles;
+    if (nchunks > frames) nchunks = frames;
+    const int64_t chunk = (frames + nchunks - 1) / nchunks;
+    nchunks = (frames + chunk - 1) / chunk;
+    k_coded_spec_fwd_cols<N, E, C><<<dim3((unsigned)(tiles * nchunks)), dim3(ColCfg<N, E>::T * C), 0, s>>>(
+        op.spec, op.h1, op.ex, op.ey, frames, op.width, tiles, (int)chunk, op.out);
+  }
   static ptx_status run(const CodedSpecCols& op, int64_t frames, hipStream_t s) {
-    launch_cols<N, E, +1>(op, frames, op.width, s);
+    constexpr int CM = ColCfg<N, E>::CMAX;
+    bool done = false;
+    if constexpr (CM > 8) {
+      if (op.width < CM) {  // frames narrower than CMAX columns: tiles of 8
+        launch<8>(op, frames, s);
+        done = true;
+      }
+    }
+    if (!done) launch<CM>(op, frames, s);
     return check_launch("coded spectrum cols", s);
   }
 };
