@@ -2,7 +2,7 @@
 the FP64 oracle and the unfused torch path on the same GPU.
 
 Tolerance: norm-wise relative error <= 1e-5 in FP32 for amplitudes, per-position
-losses and each gradient (the ``north_star`` bar); the FFT alone <= 2e-6.
+losses and each gradient; the FFT alone <= 2e-6.
 """
 
 from __future__ import annotations
@@ -67,7 +67,7 @@ def test_base_forward_backward_vs_oracle(gpu, H, h, B):
 
 @pytest.mark.parametrize("det", [False, True])
 def test_base_native_vs_torch_path_full_size(gpu, det):
-    """BASELINE configuration 2 shapes: object 2048^2, probe 512^2, B=64."""
+    """The headline workload's shapes: object 2048^2, probe 512^2, B=64."""
     from ptychox_amd import ptychoX_base, set_scatter_mode
 
     rng = np.random.default_rng(2)
