@@ -96,7 +96,7 @@ struct StepColsShape {
   static constexpr size_t LDS_BYTES = (2 * (size_t)C * col_stride<N, C>() + N) * sizeof(float2);
 };
 
-template <int N, int E, int C>
+template <int N, int E, int C, bool AMP>
 __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     k_base_step_cols(float2* __restrict__ tb, const float* __restrict__ measured, const int64_t* __restrict__ order,
                      int64_t count, float* __restrict__ amp_out, float* __restrict__ loss_part, int64_t width,
@@ -163,12 +163,14 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
       v[q] = c_scale(v[q], gscale * d * fast_rcp(fmaxf(a, kFltEps)));
       m[q] = a;
     }
-    // amplitudes (dropped by a zero-length buffer when not requested) and the
-    // wave's loss partial (lane 0; the other lanes' offsets fall outside the
-    // 4-byte buffer): stores without branches, for the same reason as above
-    const auto ra = buffer_rsrc(amp_out ? amp_out + ti.frame * fsz : measured, amp_out ? fbytes / 2 : 0u);
+    // amplitudes (AMP: requested) and the wave's loss partial (lane 0; the
+    // other lanes' offsets fall outside the 4-byte buffer): stores without
+    // branches, for the same reason as above
+    if constexpr (AMP) {
+      const auto ra = buffer_rsrc(amp_out + ti.frame * fsz, fbytes / 2);
 #pragma unroll
-    for (int q = 0; q < E; ++q) buf_store_f1(m[q], ra, lane * 4u, (q * rstep + col0) * 4u);
+      for (int q = 0; q < E; ++q) buf_store_f1(m[q], ra, lane * 4u, (q * rstep + col0) * 4u);
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
       if (off < Sh::THREADS) lsum += __shfl_xor(lsum, off, 64);
@@ -247,8 +249,12 @@ inline void launch_step_cols(float2* tb, const float* measured, const int64_t* o
   const int by_lds = (int)((160 * 1024) / Sh::LDS_BYTES);
   if (by_lds < per_cu) per_cu = by_lds;
   const int64_t grid = persistent_grid(frames * tiles, per_cu);
-  k_base_step_cols<N, E, C><<<dim3((unsigned)grid), dim3(Sh::T * C), 0, s>>>(
-      tb, measured, order, count, amp, loss_part, width, tiles, frames * tiles, gscale);
+  if (amp)
+    k_base_step_cols<N, E, C, true><<<dim3((unsigned)grid), dim3(Sh::T * C), 0, s>>>(
+        tb, measured, order, count, amp, loss_part, width, tiles, frames * tiles, gscale);
+  else
+    k_base_step_cols<N, E, C, false><<<dim3((unsigned)grid), dim3(Sh::T * C), 0, s>>>(
+        tb, measured, order, count, amp, loss_part, width, tiles, frames * tiles, gscale);
 }
 
 template <int N, int E>
