@@ -124,8 +124,8 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
                     float* __restrict__ loss_part, int64_t width, int64_t tiles_per_frame, float gscale, float inv_n) {
   using Cf = ColCfg<N, E>;
   constexpr int T = Cf::T;
+  constexpr int THREADS = T * C, WAVES = (THREADS + 63) / 64;
   __shared__ float2 lds[C * col_stride<N, C>()];
-  __shared__ float red[16];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
   const int64_t blk = xcd_paired_block(blockIdx.x, gridDim.x);
@@ -148,8 +148,12 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     if (amp_out) amp_out[k] = a;
     v[q] = c_scale(v[q], gscale * d * inv_n * fast_rcp(fmaxf(a, kFltEps)));
   }
-  const float tot = block_sum(lsum, red);
-  if (threadIdx.x == 0) loss_part[frame * tiles_per_frame + tile] = tot;
+  // one loss partial per wave (no work-group barrier)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    if (o < THREADS) lsum += __shfl_xor(lsum, o, 64);
+  if ((threadIdx.x & 63) == 0) loss_part[(frame * tiles_per_frame + tile) * WAVES + (threadIdx.x >> 6)] = lsum;
+  lds_barrier();  // the second transform rewrites the LDS
   fft_seq<N, E, -1>(v, t, &lds[c * col_stride<N, C>()]);
 #pragma unroll
   for (int q = 0; q < E; ++q) tb[off + ((t + q * T) * (int)width + (int)col)] = v[q];
@@ -305,10 +309,12 @@ struct FpmStepColsRun {
     return check_launch("fpm step cols", s);
   }
 };
+// loss partials per frame of k_fpm_step_cols: one per wave of each column tile
 template <int N, int E>
-struct ColTiles {
-  static ptx_status run(int64_t width, int64_t* tiles) {
-    *tiles = width >= ColCfg<N, E>::CMAX ? width / ColCfg<N, E>::CMAX : width / 8;
+struct LossPartsQuery {
+  static ptx_status run(int64_t width, int64_t* parts) {
+    const int c = width >= ColCfg<N, E>::CMAX ? ColCfg<N, E>::CMAX : 8;
+    *parts = (width / c) * ((ColCfg<N, E>::T * c + 63) / 64);
     return PTX_OK;
   }
 };
@@ -347,20 +353,20 @@ struct FpmWorkspace {
   float2* gspec;     // [H, W] spectrum gradient
   float2* tb;        // [B, h, w]
   float2* gq_part;   // [nchunks, h, w]
-  float* loss_part;  // [B, col tiles] (step only)
+  float* loss_part;  // [B, loss partials per frame] (step only)
   GatherScratch gather;  // deterministic only
   size_t bytes;
 };
 
 FpmWorkspace carve(void* base, int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags, bool step,
-                   const Plan& p, int64_t tiles) {
+                   const Plan& p, int64_t loss_parts) {
   Carver cv{(char*)base, 0};
   FpmWorkspace ws{};
   ws.spec = step ? cv.take<float2>((size_t)(H * W)) : nullptr;
   ws.gspec = cv.take<float2>((size_t)(H * W));
   ws.tb = cv.take<float2>((size_t)(B * h * w));
   ws.gq_part = cv.take<float2>((size_t)(p.nchunks * h * w));
-  ws.loss_part = step ? cv.take<float>((size_t)(B * tiles)) : nullptr;
+  ws.loss_part = step ? cv.take<float>((size_t)(B * loss_parts)) : nullptr;
   if (use_segments(flags)) ws.gather = carve_gather(cv, H, W, h, w, B);
   ws.bytes = cv.used;
   return ws;
@@ -493,9 +499,9 @@ size_t ptx_fpm_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_
   if (check("ptx_fpm_step_workspace", H, W, h, w, B) != PTX_OK) return 0;
   Plan p{};
   dispatch_ne<PlanQuery>(PASS_ROW_LOOP, w, h, B, &p);
-  int64_t tiles = 0;
-  dispatch_ne<ColTiles>(PASS_COL, h, w, &tiles);
-  return carve(nullptr, H, W, h, w, B, flags, true, p, tiles).bytes;
+  int64_t loss_parts = 0;
+  dispatch_ne<LossPartsQuery>(PASS_COL, h, w, &loss_parts);
+  return carve(nullptr, H, W, h, w, B, flags, true, p, loss_parts).bytes;
 }
 
 ptx_status ptx_fpm_step(const void* object_field, int64_t H, int64_t W, const void* pupil, int64_t h, int64_t w,
@@ -523,9 +529,9 @@ ptx_status ptx_fpm_step(const void* object_field, int64_t H, int64_t W, const vo
   }
   Plan p{};
   dispatch_ne<PlanQuery>(PASS_ROW_LOOP, w, h, B, &p);
-  int64_t tiles = 0;
-  dispatch_ne<ColTiles>(PASS_COL, h, w, &tiles);
-  const FpmWorkspace ws = carve(workspace, H, W, h, w, B, flags, true, p, tiles);
+  int64_t loss_parts = 0;
+  dispatch_ne<LossPartsQuery>(PASS_COL, h, w, &loss_parts);
+  const FpmWorkspace ws = carve(workspace, H, W, h, w, B, flags, true, p, loss_parts);
   if (!workspace || ws_bytes < ws.bytes) {
     set_error("ptx_fpm_step: workspace of %zu bytes needed, %zu given", ws.bytes, ws_bytes);
     return PTX_ERR_WORKSPACE;
@@ -538,7 +544,7 @@ ptx_status ptx_fpm_step(const void* object_field, int64_t H, int64_t W, const vo
   const float inv_n = 1.0f / (float)(h * w);
   st = dispatch_ne<FpmStepColsRun>(PASS_COL, h, ws.tb, measured, amplitude, ws.loss_part, B, w, 2.0f * loss_scale, inv_n, s);
   if (st != PTX_OK) return st;
-  loss_rows(ws.loss_part, tiles, B, inv_n, loss_per_position, s);
+  loss_rows(ws.loss_part, loss_parts, B, inv_n, loss_per_position, s);
   if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
   return fpm_tail(ws, p, ws.spec, H, W, (const float2*)pupil, h, w, led, B, (float2*)grad_object,
                   (float2*)grad_pupil, flags, s);
