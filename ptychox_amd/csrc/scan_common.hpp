@@ -6,6 +6,7 @@
 
 #include <type_traits>
 
+#include "buffer_io.hpp"
 #include "common_kernels.hpp"
 #include "passes.hpp"
 #include "ptx_internal.hpp"
@@ -47,6 +48,91 @@ struct ScanGatherRows {
     for (int q = 0; q < E; ++q) d[t + q * T] = v[q];
   }
 };
+
+// ---------------------------------------------------------------- the same gather, probe rows resident
+// Work-group (row group, chunk of views): S probe rows (with the fftshift sign
+// folded in) stay in registers while the work-group walks its views, so the
+// probe is read once per chunk instead of once per view.  The views are
+// software-pipelined: two register sets alternate, the next view's window
+// row in flight during this view's transform.  Loads and stores are raw
+// buffer accesses without branches: an invalid window, a row past h or a
+// view past the chunk reads zero through a zero-length buffer or an
+// out-of-range offset, and such rows are stored to an out-of-range offset
+// (dropped), so the compiler's wait counts keep the prefetch in flight.  The
+// exchanges alternate two LDS regions.  Same arithmetic per element as
+// ScanGatherRows.  Needs the object below 4 GiB (32-bit buffer offsets).
+template <int N, int E, bool SIGN>
+__global__ void __launch_bounds__(256)
+    k_scan_gather_loop(const float2* __restrict__ sample, int64_t H, int64_t W, const float2* __restrict__ probe,
+                       int h, const int64_t* __restrict__ pos, int64_t B, int chunk, int ngroups,
+                       float2* __restrict__ out) {
+  using Cf = RowCfg<N, E>;
+  constexpr int T = Cf::T, S = Cf::S;
+  constexpr bool ODD = (FftShape<N, E>::STAGES - 1) % 2 == 1;
+  __shared__ float2 lds0[S * lds_len<N>()];
+  __shared__ float2 lds1[S * lds_len<N>()];
+  __shared__ float2 s_tw[N];
+  load_lds_twiddles<N>(s_tw);
+  const int seq = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int group = blockIdx.x % ngroups;
+  const unsigned ch = blockIdx.x / ngroups;
+  const int r = group * S + seq;
+  const bool rv = r < h;
+  const unsigned b0 = ch * (unsigned)chunk;  // < B
+  const unsigned b1 = b0 + chunk < (unsigned)B ? b0 + chunk : (unsigned)B;
+  const uint32_t obj_bytes = (uint32_t)(H * W * 8);
+  const uint32_t frame_bytes = (uint32_t)((int64_t)h * N * 8);
+  constexpr uint32_t kOut = 0x80000000u;  // an offset past every buffer: reads zero, drops stores
+  float2 pr[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int i = t + q * T;
+    const float2 p = rv ? probe[(int64_t)r * N + i] : c_zero();
+    pr[q] = (SIGN && ((r + i) & 1)) ? c_neg(p) : p;
+  }
+  const uint32_t out_lane = rv ? (uint32_t)(((int64_t)r * N + t) * 8) : kOut;
+  auto fetch = [&](unsigned b, float2 (&dv)[E]) {
+    const bool live = b < b1;
+    const unsigned bb = live ? b : b0;
+    const int64_t x0 = pos[2 * bb], y0 = pos[2 * bb + 1];
+    const bool ok = live && window_ok(x0, y0, h, N, H, W);
+    const auto rs = buffer_rsrc(sample, ok ? obj_bytes : 0u);
+    const uint32_t voff = (ok && rv) ? (uint32_t)(((y0 + r) * W + x0 + t) * 8) : kOut;
+#pragma unroll
+    for (int q = 0; q < E; ++q) dv[q] = buf_load_f2(rs, voff, (uint32_t)(q * T * 8));
+  };
+  auto process = [&](unsigned b, float2 (&v)[E], bool second) {
+    int tt = t, sq = seq;
+    asm volatile("" : "+v"(tt), "+v"(sq));  // exchange addresses per view, not hoisted into registers
+    float2* l0 = &lds0[sq * lds_len<N>()];
+    float2* l1 = &lds1[sq * lds_len<N>()];
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = c_mul(v[q], pr[q]);
+    if (second)
+      fft_stages<N, E, -1, true, 0, true>(v, tt, l1, l0, s_tw);
+    else
+      fft_stages<N, E, -1, true, 0, true>(v, tt, l0, l1, s_tw);
+    const auto ro = buffer_rsrc(out + (int64_t)b * h * N, frame_bytes);
+#pragma unroll
+    for (int q = 0; q < E; ++q) buf_store_f2(v[q], ro, out_lane, (uint32_t)(q * T * 8));
+  };
+  float2 va[E], vb[E];
+  unsigned b = b0;
+  fetch(b, va);
+  fetch(b + 1, vb);
+  process(b, va, false);
+  ++b;
+  while (b < b1) {
+    fetch(b + 1, va);
+    process(b, vb, ODD);
+    ++b;
+    if (b >= b1) break;
+    fetch(b + 1, vb);
+    process(b, va, false);
+    ++b;
+  }
+}
 
 // ---------------------------------------------------------------- row iFFT, (sign | scale), object scatter, probe partials
 template <int N, int E, bool SIGN>
@@ -115,7 +201,20 @@ template <int N, int E>
 struct GatherRowsRun {
   template <bool SIGN>
   static ptx_status run(const ScanGatherRows<SIGN>& op, int64_t nrows, hipStream_t s) {
-    launch_rows<N, E, -1>(op, nrows, s);
+    using Cf = RowCfg<N, E>;
+    const int64_t B = op.h > 0 ? nrows / op.h : 0;
+    if (B > 0 && op.H * op.W * 8 < (int64_t(1) << 31) && (int64_t)op.h * N * 8 < (int64_t(1) << 31)) {
+      // probe rows resident: ~1024 work-groups, each walking a chunk of views
+      const int ngroups = (int)((op.h + Cf::S - 1) / Cf::S);
+      int64_t nchunks = (1024 + ngroups - 1) / ngroups;
+      if (nchunks > B) nchunks = B;
+      const int64_t chunk = (B + nchunks - 1) / nchunks;
+      nchunks = (B + chunk - 1) / chunk;
+      k_scan_gather_loop<N, E, SIGN><<<dim3((unsigned)(ngroups * nchunks)), dim3(256), 0, s>>>(
+          op.sample, op.H, op.W, op.probe, op.h, op.pos, B, (int)chunk, ngroups, op.out);
+    } else {
+      launch_rows<N, E, -1>(op, nrows, s);
+    }
     return check_launch(SIGN ? "base gather rows" : "scan gather rows", s);
   }
 };
