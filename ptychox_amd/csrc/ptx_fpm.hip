@@ -18,6 +18,7 @@
 // scale/sign + amplitude | C2 Gamma sign + column FFT, R2 row FFT + pupil
 // partials + spectrum scatter (atomics or sorted segments), spectrum iFFT (rows,
 // cols).  The step fuses C1 and C2 through the loss.
+#include "buffer_io.hpp"
 #include "common_kernels.hpp"
 #include "passes.hpp"
 #include "ptx_internal.hpp"
@@ -57,6 +58,82 @@ struct FpmGatherRows {
     for (int q = 0; q < E; ++q) d[t + q * T] = v[q];
   }
 };
+
+// The same gather with the pupil rows resident: work-group (row group, chunk
+// of LEDs) keeps S pupil rows in registers and walks its LEDs, the next LED's
+// spectrum row in flight during this LED's transform (two register sets);
+// every element's spectrum column wraps modulo W (a power of two), so each
+// element has its own buffer offset; rows past h read zero and are stored to
+// an out-of-range offset (dropped) -- no branches around memory accesses.
+// The exchanges alternate two LDS regions.  Needs the spectrum below 2 GiB.
+template <int N, int E>
+__global__ void __launch_bounds__(256)
+    k_fpm_gather_loop(const float2* __restrict__ spec, int64_t H, int64_t W, const float2* __restrict__ pupil, int h,
+                      const int64_t* __restrict__ led, int64_t B, int chunk, int ngroups, float2* __restrict__ out) {
+  using Cf = RowCfg<N, E>;
+  constexpr int T = Cf::T, S = Cf::S;
+  constexpr bool ODD = (FftShape<N, E>::STAGES - 1) % 2 == 1;
+  __shared__ float2 lds0[S * lds_len<N>()];
+  __shared__ float2 lds1[S * lds_len<N>()];
+  __shared__ float2 s_tw[N];
+  load_lds_twiddles<N>(s_tw);
+  const int seq = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int group = blockIdx.x % ngroups;
+  const unsigned ch = blockIdx.x / ngroups;
+  const int r = group * S + seq;
+  const bool rv = r < h;
+  const unsigned b0 = ch * (unsigned)chunk;  // < B
+  const unsigned b1 = b0 + chunk < (unsigned)B ? b0 + chunk : (unsigned)B;
+  const uint32_t spec_bytes = (uint32_t)(H * W * 8);
+  const uint32_t frame_bytes = (uint32_t)((int64_t)h * N * 8);
+  constexpr uint32_t kOut = 0x80000000u;  // an offset past every buffer: reads zero, drops stores
+  const auto rs = buffer_rsrc(spec, spec_bytes);
+  float2 pr[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) pr[q] = rv ? pupil[(int64_t)r * N + t + q * T] : c_zero();
+  const uint32_t out_lane = rv ? (uint32_t)(((int64_t)r * N + t) * 8) : kOut;
+  auto fetch = [&](unsigned b, float2 (&dv)[E]) {
+    const unsigned bb = b < b1 ? b : b0;  // past the chunk: a valid LED's row, never used
+    const int64_t sy = (led[2 * bb + 1] + r + H / 2) & (H - 1);
+    const int64_t sx0 = led[2 * bb] + W / 2;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const uint32_t off = rv ? (uint32_t)((sy * W + ((sx0 + t + q * T) & (W - 1))) * 8) : kOut;
+      dv[q] = buf_load_f2(rs, off, 0u);
+    }
+  };
+  auto process = [&](unsigned b, float2 (&v)[E], bool second) {
+    int tt = t, sq = seq;
+    asm volatile("" : "+v"(tt), "+v"(sq));  // exchange addresses per LED, not hoisted into registers
+    float2* l0 = &lds0[sq * lds_len<N>()];
+    float2* l1 = &lds1[sq * lds_len<N>()];
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = c_mul(v[q], pr[q]);
+    if (second)
+      fft_stages<N, E, +1, true, 0, true>(v, tt, l1, l0, s_tw);
+    else
+      fft_stages<N, E, +1, true, 0, true>(v, tt, l0, l1, s_tw);
+    const auto ro = buffer_rsrc(out + (int64_t)b * h * N, frame_bytes);
+#pragma unroll
+    for (int q = 0; q < E; ++q) buf_store_f2(v[q], ro, out_lane, (uint32_t)(q * T * 8));
+  };
+  float2 va[E], vb[E];
+  unsigned b = b0;
+  fetch(b, va);
+  fetch(b + 1, vb);
+  process(b, va, false);
+  ++b;
+  while (b < b1) {
+    fetch(b + 1, va);
+    process(b, vb, ODD);
+    ++b;
+    if (b >= b1) break;
+    fetch(b + 1, vb);
+    process(b, va, false);
+    ++b;
+  }
+}
 
 // ---------------------------------------------------------------- C1: column iFFT -> image field, amplitude
 struct FpmImageCols {
@@ -272,7 +349,20 @@ struct ObjFftColsOut {
 template <int N, int E>
 struct FpmGatherRun {
   static ptx_status run(const FpmGatherRows& op, int64_t nrows, hipStream_t s) {
-    launch_rows<N, E, +1>(op, nrows, s);
+    using Cf = RowCfg<N, E>;
+    const int64_t B = op.h > 0 ? nrows / op.h : 0;
+    if (B > 0 && op.H * op.W * 8 < (int64_t(1) << 31) && (int64_t)op.h * N * 8 < (int64_t(1) << 31)) {
+      // pupil rows resident: ~1024 work-groups, each walking a chunk of LEDs
+      const int ngroups = (int)((op.h + Cf::S - 1) / Cf::S);
+      int64_t nchunks = (1024 + ngroups - 1) / ngroups;
+      if (nchunks > B) nchunks = B;
+      const int64_t chunk = (B + nchunks - 1) / nchunks;
+      nchunks = (B + chunk - 1) / chunk;
+      k_fpm_gather_loop<N, E><<<dim3((unsigned)(ngroups * nchunks)), dim3(256), 0, s>>>(
+          op.spec, op.H, op.W, op.pupil, op.h, op.led, B, (int)chunk, ngroups, op.out);
+    } else {
+      launch_rows<N, E, +1>(op, nrows, s);
+    }
     return check_launch("fpm gather rows", s);
   }
 };
