@@ -128,3 +128,48 @@ def test_fpm_segmented_gather_vs_oracle(gpu, H, h, n_side):
         assert max(e) < TOL
         got[mode] = ot.grad
     assert nrel(got["segments"], got["atomic"]) < TOL
+
+
+# The forward gather passes walk a chunk of views per work-group.  With
+# 1024-point rows (8 elements per thread: three LDS exchanges per transform)
+# and 4 views, each work-group transforms two views, the second starting on the
+# other LDS region: parity of that alternation against the oracle.
+def test_base_gather_two_views_per_group_odd_exchanges(gpu):
+    from ptychox_amd import ptychoX_base
+
+    rng = np.random.default_rng(21)
+    H, h, B = 1100, 1024, 4
+    s = (smooth_object(rng, H) + 0.05 * crandn(rng, H, H)).astype(np.complex64)
+    p = disc_probe(h, h / 4)
+    pos = rng.integers(0, H - h + 1, (B, 2)).astype(np.int64)
+    g = rng.standard_normal((B, h, h)).astype(np.float32)
+    st, pt = to_dev(s, gpu).requires_grad_(True), to_dev(p, gpu).requires_grad_(True)
+    amp = ptychoX_base(st, pt, to_dev(pos, gpu), use_cuda_kernels=True)
+    amp.backward(to_dev(g, gpu))
+    fwd = po.base_forward(s, p, pos)
+    gs, gp = po.base_backward(g, s.shape, p, fwd["patches"], fwd["far_field"], pos)
+    e = (nrel(amp, fwd["amplitude"]), nrel(st.grad, gs), nrel(pt.grad, gp))
+    print(f"base {H}/{h}/B{B}: amp {e[0]:.2e} grad_sample {e[1]:.2e} grad_probe {e[2]:.2e} (vs FP64 oracle)",
+          flush=True)
+    assert max(e) < TOL
+
+
+def test_fpm_gather_two_views_per_group_odd_exchanges(gpu):
+    from ptychox_amd import ptychoX_fourier
+
+    rng = np.random.default_rng(22)
+    H, h = 1024, 1024
+    o = (smooth_object(rng, H, 3.0) + 0.05 * crandn(rng, H, H)).astype(np.complex64)
+    q = _pupil(h, h * 0.375, rng)
+    led = np.array([[0, 0], [3, -2], [-5, 7], [1, 1]], dtype=np.int64)
+    B = led.shape[0]
+    g = rng.standard_normal((B, h, h)).astype(np.float32)
+    ot, qt = to_dev(o, gpu).requires_grad_(True), to_dev(q, gpu).requires_grad_(True)
+    amp = ptychoX_fourier(ot, qt, to_dev(led, gpu), use_cuda_kernels=True)
+    amp.backward(to_dev(g, gpu))
+    fwd = po.fpm_forward(o, q, led)
+    go, gq = po.fpm_backward(g, o.shape, q, fwd["sub_spectrum"], fwd["image_field"], led)
+    e = (nrel(amp, fwd["amplitude"]), nrel(ot.grad, go), nrel(qt.grad, gq))
+    print(f"fpm {H}/{h}/B{B}: amp {e[0]:.2e} grad_object {e[1]:.2e} grad_pupil {e[2]:.2e} (vs FP64 oracle)",
+          flush=True)
+    assert max(e) < TOL
