@@ -100,6 +100,18 @@ const char* ptx_timing_label(int k);
 ptx_status ptx_tune_set(int kind, int64_t n, int elems);
 int ptx_tune_get(int kind, int64_t n);
 
+/* Work-group target of the passes that walk a chunk of views per work-group,
+ * per pass kind and transform length: kind 0 the forward gathers (default
+ * 1024), 1 the sorted-segment scatters (1024), 2 the coded spectrum column
+ * pass (256), 3 the coded spectrum-gradient column pass (256), 4 the coded
+ * mask-gradient row pass (2048).  The chunk of views per work-group follows
+ * from it; for kinds 1, 3 and 4 so does the number of partial slices, hence
+ * the workspace size (query the *_workspace functions after changing it).
+ * groups: 1..65536 (1: every work-group walks all views of its rows or
+ * columns), 0 restores the default.  No reference counterpart. */
+ptx_status ptx_tune_set_groups(int kind, int64_t n, int groups);
+int ptx_tune_get_groups(int kind, int64_t n); /* 0: the built-in default */
+
 /* ------------------------------------------------------------------ FFT
  * Batched in-place 2-D C2C transform of ``batch`` frames of h x w,
  * unnormalised in both directions (inverse != 0 uses exp(+i)).

@@ -508,7 +508,7 @@ struct SpecColsRun {
   static void launch(const CodedSpecCols& op, int64_t frames, hipStream_t s) {
     if (frames <= 0) return;
     const int64_t tiles = op.width / C;
-    int64_t nchunks = (256 + tiles - 1) / tiles;
+    int64_t nchunks = (tuned_groups(PASS_COL, N, 256) + tiles - 1) / tiles;
     if (nchunks > frames) nchunks = frames;
     const int64_t chunk = (frames + nchunks - 1) / nchunks;
     nchunks = (frames + chunk - 1) / chunk;
@@ -578,23 +578,22 @@ template <int N, int E>
 struct RowChunkQuery {
   static ptx_status run(int64_t H, int64_t B, Chunks* c) {
     c->row_groups = (int)((H + RowCfg<N, E>::S - 1) / RowCfg<N, E>::S);
-    int64_t want = (2048 + c->row_groups - 1) / c->row_groups;
+    int64_t want = (tuned_groups(PASS_VIEW_ROWS, N, 2048) + c->row_groups - 1) / c->row_groups;
     want = want < 1 ? 1 : (want > B ? B : want);
     c->row_chunk = (int)((B + want - 1) / want);
     c->row_nchunks = (int)((B + c->row_chunk - 1) / c->row_chunk);
     return PTX_OK;
   }
 };
-// Q4 chunks: about 256 work-groups (one per CU of the target, one resident
-// work-group each), so each walks a long chunk of views and few partial
-// spectra are written and summed.  A fixed target keeps the workspace size a
-// function of the shapes alone.
+// Q4 chunks: about 256 work-groups by default (one per CU of the target, one
+// resident work-group each; tunable), so each walks a long chunk of views and
+// few partial spectra are written and summed.
 template <int N, int E>
 struct ColChunkQuery {
   static ptx_status run(int64_t W, int64_t B, Chunks* c) {
     c->col_c = W >= ColCfg<N, E>::CMAX ? ColCfg<N, E>::CMAX : 8;
     c->col_tiles = (int)(W / c->col_c);
-    int64_t want = (256 + c->col_tiles - 1) / c->col_tiles;
+    int64_t want = (tuned_groups(PASS_COL_LOOP, N, 256) + c->col_tiles - 1) / c->col_tiles;
     want = want < 1 ? 1 : (want > B ? B : want);
     c->col_chunk = (int)((B + want - 1) / want);
     c->col_nchunks = (int)((B + c->col_chunk - 1) / c->col_chunk);

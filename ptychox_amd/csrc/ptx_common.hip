@@ -26,6 +26,15 @@ static int default_elems(int kind, int64_t n) {
   return kind == PASS_ROW_LOOP ? 16 : 8;  // PASS_VIEW_ROWS: 8 (five register arrays per thread)
 }
 
+static std::atomic<int> g_groups[PASS_KINDS][16];  // 0 = built-in default
+
+int tuned_groups(int kind, int64_t n, int def) {
+  if (kind < 0 || kind >= PASS_KINDS || !is_pow2(n)) return def;
+  const int l = ilog2((int)n);
+  const int v = (l >= 0 && l < 16) ? g_groups[kind][l].load(std::memory_order_relaxed) : 0;
+  return v ? v : def;
+}
+
 int tuned_elems(int kind, int64_t n) {
   if (kind < 0 || kind >= PASS_KINDS || !is_pow2(n)) return default_elems(kind, n);
   const int l = ilog2((int)n);
@@ -139,6 +148,25 @@ ptx_status ptx_tune_set(int kind, int64_t n, int elems) {
 }
 
 int ptx_tune_get(int kind, int64_t n) { return size_ok(n) ? tuned_elems(kind, n) : 0; }
+
+ptx_status ptx_tune_set_groups(int kind, int64_t n, int groups) {
+  if (kind < 0 || kind >= PASS_KINDS) {
+    set_error("ptx_tune_set_groups: pass kind %d out of range [0, %d)", kind, (int)PASS_KINDS);
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  if (!size_ok(n)) {
+    set_error("ptx_tune_set_groups: length %lld unsupported", (long long)n);
+    return PTX_ERR_UNSUPPORTED;
+  }
+  if (groups < 0 || groups > 65536) {
+    set_error("ptx_tune_set_groups: work-group target must be in [1, 65536] (0 = default), got %d", groups);
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  g_groups[kind][ilog2((int)n)].store(groups, std::memory_order_relaxed);
+  return PTX_OK;
+}
+
+int ptx_tune_get_groups(int kind, int64_t n) { return size_ok(n) ? tuned_groups(kind, n, 0) : 0; }
 
 const char* ptx_timing_label(int k) { return (k >= 0 && k < t_used) ? t_labels[k] : ""; }
 

@@ -61,6 +61,12 @@ enum PassKind { PASS_ROW = 0, PASS_ROW_LOOP = 1, PASS_COL = 2, PASS_COL_LOOP = 3
 // 256..1024 points, the view-looping row pass prefers radix 16).
 int tuned_elems(int kind, int64_t n);
 
+// Work-group target of the passes that walk views (the gathers, the
+// sorted-segment scatters and the coded view loops): how many work-groups the
+// views of a launch are spread over, which fixes the chunk of views each
+// work-group walks.  `def` when the table holds no setting for (kind, n).
+int tuned_groups(int kind, int64_t n, int def);
+
 // Call F<N, E>::run(args...) for the runtime power-of-two n and the tuned E.
 template <template <int, int> class F, typename... Args>
 ptx_status dispatch_ne(int kind, int64_t n, Args&&... args) {

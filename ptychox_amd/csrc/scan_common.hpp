@@ -204,9 +204,9 @@ struct GatherRowsRun {
     using Cf = RowCfg<N, E>;
     const int64_t B = op.h > 0 ? nrows / op.h : 0;
     if (B > 0 && op.H * op.W * 8 < (int64_t(1) << 31) && (int64_t)op.h * N * 8 < (int64_t(1) << 31)) {
-      // probe rows resident: ~1024 work-groups, each walking a chunk of views
+      // probe rows resident: ~1024 work-groups (tunable), each walking a chunk of views
       const int ngroups = (int)((op.h + Cf::S - 1) / Cf::S);
-      int64_t nchunks = (1024 + ngroups - 1) / ngroups;
+      int64_t nchunks = (tuned_groups(PASS_ROW, N, 1024) + ngroups - 1) / ngroups;
       if (nchunks > B) nchunks = B;
       const int64_t chunk = (B + nchunks - 1) / nchunks;
       nchunks = (B + chunk - 1) / chunk;
@@ -231,8 +231,8 @@ struct ScatterPlanQuery {
   static ptx_status run(int64_t h, int64_t B, ScatterPlan* p) {
     p->S = RowCfg<N, E>::S;
     p->ngroups = (int)((h + p->S - 1) / p->S);
-    // aim for ~1024 work-groups; each probe-partial slice covers `chunk` views
-    int64_t want = (1024 + p->ngroups - 1) / p->ngroups;
+    // aim for ~1024 work-groups (tunable); each probe-partial slice covers `chunk` views
+    int64_t want = (tuned_groups(PASS_ROW_LOOP, N, 1024) + p->ngroups - 1) / p->ngroups;
     if (want < 1) want = 1;
     if (want > B) want = B > 0 ? B : 1;
     p->chunk = (int)((B + want - 1) / want);

@@ -121,6 +121,29 @@ def test_tuning_table_roundtrip(lib):
     from ptychox_amd import autotune
 
     t = autotune.table(autotune.keys_for_scan((512, 512)))
-    assert set(t) == {"row:512", "row_loop:512", "col:512"}
+    assert set(t) == {"row:512", "row_loop:512", "col:512", "row:512@groups", "row_loop:512@groups",
+                      "col:512@groups"}
     autotune.apply(t)
     assert autotune.table(autotune.keys_for_scan((512, 512))) == t
+
+
+def test_group_table_roundtrip(lib):
+    h = lib.lib()
+    assert h.ptx_tune_get_groups(1, 512) == 0  # built-in default
+    lib.call("ptx_tune_set_groups", 1, 512, 64)
+    assert h.ptx_tune_get_groups(1, 512) == 64
+    # the scatter's work-group target fixes its probe-partial slices, hence the workspace
+    before = lib.query("ptx_base_step_workspace", 2048, 2048, 512, 512, 64, 0)
+    lib.call("ptx_tune_set_groups", 1, 512, 4096)
+    after = lib.query("ptx_base_step_workspace", 2048, 2048, 512, 512, 64, 0)
+    assert after > before
+    lib.call("ptx_tune_set_groups", 1, 512, 0)
+    assert h.ptx_tune_get_groups(1, 512) == 0
+    with pytest.raises(ValueError):
+        lib.call("ptx_tune_set_groups", 5, 512, 64)
+    with pytest.raises(ValueError):
+        lib.call("ptx_tune_set_groups", 0, 512, -1)
+    with pytest.raises(ValueError):
+        lib.call("ptx_tune_set_groups", 0, 512, 1 << 20)
+    with pytest.raises(lib.NativeLibraryError, match="unsupported"):
+        lib.call("ptx_tune_set_groups", 0, 12, 64)

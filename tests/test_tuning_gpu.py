@@ -1,6 +1,7 @@
 """Every operator stays in parity with the FP64 oracle under both compiled
-FFT schedules (8 or 16 elements per thread in every pass kind), and the
-autotuner leaves a valid table behind."""
+FFT schedules (8 or 16 elements per thread in every pass kind) and under
+extreme work-group targets of the view-walking passes, and the autotuner
+leaves a valid table behind."""
 
 from __future__ import annotations
 
@@ -17,25 +18,30 @@ TOL = 1e-5
 SIZES = (16, 32, 64, 128, 256, 512, 1024, 2048)
 
 
-def _force(elems):
+def _force(elems, groups=0):
     from ptychox_amd import _lib
 
     for kind in range(5):
         for n in SIZES:
             _lib.call("ptx_tune_set", kind, n, elems)
+            _lib.call("ptx_tune_set_groups", kind, n, groups)
 
 
-@pytest.fixture(params=[8, 16])
+# (elements per thread, work-group target): target 1 makes every work-group of
+# the view-walking passes walk all views of its rows / columns (the pipelined
+# view loops, with the alternating LDS regions, at these small shapes); 65536
+# leaves one view per work-group
+@pytest.fixture(params=[(8, 0), (16, 0), (8, 1), (16, 1), (8, 65536)])
 def schedule(request, gpu):
-    _force(request.param)
+    _force(*request.param)
     yield request.param
-    _force(0)
+    _force(0, 0)
 
 
 def test_all_operators_under_schedule(gpu, schedule):
     from ptychox_amd import ptychoX_base, ptychoX_coded, ptychoX_fourier, ptychoX_nearfield
 
-    rng = np.random.default_rng(schedule)
+    rng = np.random.default_rng(schedule[0] + schedule[1])
     # conventional
     s = smooth_object(rng, 160)
     p = disc_probe(64, 16)
@@ -75,7 +81,7 @@ def test_all_operators_under_schedule(gpu, schedule):
     f = po.coded_forward(o, c, t1, t2, sh.astype(np.float64), 2)
     go, gcc = po.coded_backward(gc, f, c, t1, t2, 2)
     e = (nrel(ot.grad, go), nrel(ct.grad, gcc))
-    print(f"schedule E={schedule}: coded {e[0]:.2e} {e[1]:.2e}", flush=True)
+    print(f"schedule (E, groups)={schedule}: coded {e[0]:.2e} {e[1]:.2e}", flush=True)
     assert max(e) < TOL
 
 
@@ -91,8 +97,19 @@ def test_autotune_picks_valid_table(gpu):
     gs, gp = torch.empty_like(s), torch.empty_like(p)
     step = BaseStep(s.shape, p.shape, 16, gpu)
     try:
-        res = autotune.autotune(lambda: step(s, p, pos, y, gs, gp), autotune.keys_for_scan((128, 128)), reps=3)
-        assert set(res["table"].values()) <= {8, 16}
-        assert res["after_ms"] > 0 and len(res["trials"]) == 6
+        keys = autotune.keys_for_scan((128, 128))
+        res = autotune.autotune(lambda: step(s, p, pos, y, gs, gp), keys, reps=3)
+        t = res["table"]
+        assert {v for k, v in t.items() if not k.endswith(autotune.GROUPS_SUFFIX)} <= {8, 16}
+        for kind, n in keys:
+            assert t[f"{autotune.KIND_NAMES[kind]}:{n}{autotune.GROUPS_SUFFIX}"] in autotune.GROUP_CANDIDATES[kind]
+        ntrials = sum(len(autotune.CANDIDATES) + len(autotune.GROUP_CANDIDATES[k]) for k, _ in keys)
+        assert res["after_ms"] > 0 and len(res["trials"]) == ntrials
+        # the tuned step still matches the default schedule's result
+        l_tuned = step(s, p, pos, y, gs, gp).clone()
+        g_tuned = gs.clone()
+        _force(0, 0)
+        l_def = step(s, p, pos, y, gs, gp).clone()
+        assert nrel(l_tuned, l_def) < TOL and nrel(g_tuned, gs) < TOL
     finally:
-        _force(0)
+        _force(0, 0)
