@@ -102,8 +102,9 @@ int ptx_tune_get(int kind, int64_t n);
 
 /* Work-group target of the passes that walk a chunk of views per work-group,
  * per pass kind and transform length: kind 0 the forward gathers (default
- * 1024), 1 the sorted-segment scatters (1024), 2 the coded spectrum column
- * pass (256), 3 the coded spectrum-gradient column pass (256), 4 the coded
+ * 1024), 1 the sorted-segment scatters (1024), 2 the transfer-function
+ * column passes (one work-group per CU) and the coded spectrum column pass
+ * (256), 3 the coded spectrum-gradient column pass (256), 4 the coded
  * mask-gradient row pass (2048).  The chunk of views per work-group follows
  * from it; for kinds 1, 3 and 4 so does the number of partial slices, hence
  * the workspace size (query the *_workspace functions after changing it).

@@ -236,13 +236,15 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
 }
 
 // host launcher (check_launch is the caller's): tiles_per_frame x ngroups
-// work-groups, ngroups = the frame groups that fill the resident slots
+// work-groups, ngroups = the frame groups that fill the resident slots (or the
+// tuned work-group target of the column passes)
 template <int N, int E, int C>
 inline void launch_transfer_cols_c(float2* tb, const float2* transfer, int64_t frames, int64_t width, bool conj,
                                    hipStream_t s) {
   using Sh = PipeColsShape<N, E, C>;
   const int64_t tiles = width / C;
-  int64_t ngroups = (int64_t)device_cu_count() * Sh::PER_CU / tiles;
+  // work-groups: the resident slots by default, or the tuned target of the column passes
+  int64_t ngroups = (int64_t)tuned_groups(PASS_COL, N, device_cu_count() * Sh::PER_CU) / tiles;
   if (ngroups > frames) ngroups = frames;
   if (ngroups < 1) ngroups = 1;
   const dim3 grid((unsigned)(tiles * ngroups));
