@@ -109,8 +109,15 @@ inline void loss_rows(const float* part, int64_t nparts, int64_t batch, float sc
 
 // ---------------------------------------------------------------- sorted-segment scatter
 
-constexpr int kTileH = 16;   // object rows per gather tile
-constexpr int kTileW = 128;  // object columns per gather tile
+// Gather tiles are 128 object columns wide.  Their height depends on the
+// gather's mode: 16 rows when it is segmented (fewer, larger partial tiles to
+// combine); 4 rows in the single-pass mode, where a work-group's time is its
+// tile's list length, so small tiles spread a densely overlapped band of the
+// object over more work-groups (a mini-batch of consecutive raster positions
+// covers such a band: 40.0 -> 34.0 us for object 2048^2, probe 512^2, 64 views).
+constexpr int kTileW = 128;
+constexpr int kSegTileH = 16;
+constexpr int kFlatTileH = 4;
 
 __device__ __forceinline__ int64_t pmod(int64_t a, int64_t n) {
   const int64_t r = a % n;
@@ -131,12 +138,13 @@ struct TileLists {
   int64_t batch;
   int h, w;
   int64_t H, W, off_y, off_x, tiles_x;
+  int tile_h;  // tile rows
   int* lists;
   int* counts;
   int64_t ntiles;  // 0: nothing to do
   __host__ __device__ int64_t blocks() const { return ntiles; }
   __device__ __forceinline__ void run(int64_t tile, int* wave_tot /* 4 ints of LDS */) const {
-    const int64_t ty0 = (tile / tiles_x) * kTileH, tx0 = (tile % tiles_x) * kTileW;
+    const int64_t ty0 = (tile / tiles_x) * tile_h, tx0 = (tile % tiles_x) * kTileW;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int count = 0;
     for (int64_t base = 0; base < batch; base += 256) {
@@ -145,7 +153,7 @@ struct TileLists {
       if (b < batch) {
         const int64_t oy = pmod(pos[2 * b + 1] + off_y, H), ox = pmod(pos[2 * b] + off_x, W);
         // two arcs on a circle overlap iff one starts inside the other
-        const bool ry = pmod(ty0 - oy, H) < h || pmod(oy - ty0, H) < kTileH;
+        const bool ry = pmod(ty0 - oy, H) < h || pmod(oy - ty0, H) < tile_h;
         const bool rx = pmod(tx0 - ox, W) < w || pmod(ox - tx0, W) < kTileW;
         hit = ry && rx;
       }
@@ -210,19 +218,19 @@ constexpr int kSegViews = 8;
 // Views gathered per iteration: their 8-row loads are all in flight before the
 // ordered additions.
 constexpr int kGatherUnroll = 4;
-constexpr int kTilePx = kTileH * kTileW;
+constexpr int kSegTilePx = kSegTileH * kTileW;
 
 // Work items and partial-tile slots a gather can need: every tile has at
 // least one segment, and the segments beyond the first are bounded by the
 // list entries divided by kSegViews.  A view's window meets at most
-// h/16 + 2 tile rows (wrap-around on a grid of whole tiles included) and the
+// h/16 + 2 tile rows of a segmented gather (wrap-around on a grid of whole tiles included) and the
 // list test can add the grid's last, partial tile row; likewise for columns,
 // hence (h/16 + 3) x (w/128 + 3) list entries per view.
 struct GatherBounds {
   int64_t items, slots;
 };
 inline GatherBounds gather_bounds(int64_t ntiles, int64_t h, int64_t w, int64_t batch) {
-  int64_t per_view = (h / kTileH + 3) * (w / kTileW + 3);
+  int64_t per_view = (h / kSegTileH + 3) * (w / kTileW + 3);
   if (per_view > ntiles) per_view = ntiles;
   const int64_t slots = (batch * per_view + kSegViews - 1) / kSegViews;
   return GatherBounds{ntiles + slots, slots};
@@ -282,7 +290,8 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
                                                             float2* __restrict__ partial,
                                                             float2* __restrict__ grad, int64_t H, int64_t W,
                                                             int accumulate, SideWork side, int64_t main_blocks) {
-  constexpr int RPT = kTilePx / 256;  // rows per thread
+  constexpr int TH = SEG ? kSegTileH : kFlatTileH;
+  constexpr int RPT = TH * kTileW / 256;  // rows per thread
   constexpr int RSTEP = 256 / kTileW;
   constexpr int CHUNK = SEG ? kSegViews : kListChunk;
   if ((int64_t)blockIdx.x >= main_blocks) {  // work-groups riding along
@@ -303,7 +312,7 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
   } else {
     tile = blk;
   }
-  const int64_t ty0 = (tile / tiles_x) * kTileH, tx0 = (tile % tiles_x) * kTileW;
+  const int64_t ty0 = (tile / tiles_x) * TH, tx0 = (tile % tiles_x) * kTileW;
   const int64_t X = tx0 + (threadIdx.x % kTileW);
   const int64_t Y0 = ty0 + threadIdx.x / kTileW;
   const bool xin = X < W;
@@ -357,7 +366,7 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
     }
   }
   if (SEG && seg > 0) {  // a partial tile, added by the combine pass
-    float2* pt = partial + (int64_t)(slot_off[tile] + seg - 1) * kTilePx;
+    float2* pt = partial + (int64_t)(slot_off[tile] + seg - 1) * kSegTilePx;
 #pragma unroll
     for (int j = 0; j < RPT; ++j) pt[j * 256 + threadIdx.x] = acc[j];
     return;
@@ -381,23 +390,23 @@ static __global__ void __launch_bounds__(256) k_gather_combine(const int* __rest
                                                                const int* __restrict__ slot_off, int64_t tiles_x,
                                                                const float2* __restrict__ partial,
                                                                float2* __restrict__ grad, int64_t H, int64_t W) {
-  constexpr int RPT = kTilePx / 256;
+  constexpr int RPT = kSegTilePx / 256;
   constexpr int RSTEP = 256 / kTileW;
   const int64_t tile = blockIdx.x / RPT;
   const int j = blockIdx.x % RPT;
   const int n = counts[tile];
   const int nseg = n > kSegViews ? (n + kSegViews - 1) / kSegViews : 1;
   if (nseg < 2) return;
-  const int64_t ty0 = (tile / tiles_x) * kTileH, tx0 = (tile % tiles_x) * kTileW;
+  const int64_t ty0 = (tile / tiles_x) * kSegTileH, tx0 = (tile % tiles_x) * kTileW;
   const int64_t X = tx0 + (threadIdx.x % kTileW);
   const int64_t Y = ty0 + threadIdx.x / kTileW + j * RSTEP;
   if (X >= W || Y >= H) return;
-  const float2* pt = partial + (int64_t)slot_off[tile] * kTilePx + j * 256 + threadIdx.x;
+  const float2* pt = partial + (int64_t)slot_off[tile] * kSegTilePx + j * 256 + threadIdx.x;
   float2 s = grad[Y * W + X];
   for (int sg = 0; sg < nseg - 1; sg += kCombineBatch) {
     float2 v[kCombineBatch];
 #pragma unroll
-    for (int k = 0; k < kCombineBatch; ++k) v[k] = sg + k < nseg - 1 ? pt[(int64_t)(sg + k) * kTilePx] : c_zero();
+    for (int k = 0; k < kCombineBatch; ++k) v[k] = sg + k < nseg - 1 ? pt[(int64_t)(sg + k) * kSegTilePx] : c_zero();
 #pragma unroll
     for (int k = 0; k < kCombineBatch; ++k)
       if (sg + k < nseg - 1) s = c_add(s, v[k]);
@@ -407,11 +416,9 @@ static __global__ void __launch_bounds__(256) k_gather_combine(const int* __rest
 
 struct TileGrid {
   int64_t tiles_x, tiles_y;
+  int tile_h;
   int64_t count() const { return tiles_x * tiles_y; }
 };
-inline TileGrid tile_grid(int64_t H, int64_t W) {
-  return TileGrid{(W + kTileW - 1) / kTileW, (H + kTileH - 1) / kTileH};
-}
 
 // ---------------------------------------------------------------- workspace carving
 
@@ -432,6 +439,10 @@ struct Carver {
 inline bool gather_segmented(int64_t h, int64_t w, int64_t B, int64_t H, int64_t W) {
   return B * h * w >= (int64_t)kSegViews * H * W;
 }
+inline TileGrid tile_grid(int64_t h, int64_t w, int64_t B, int64_t H, int64_t W) {
+  const int th = gather_segmented(h, w, B, H, W) ? kSegTileH : kFlatTileH;
+  return TileGrid{(W + kTileW - 1) / kTileW, (H + th - 1) / th, th};
+}
 
 // Scratch of the deterministic gradient gather: per-tile view lists, the
 // segment plan and the partial tiles of segments beyond the first.
@@ -442,11 +453,11 @@ struct GatherScratch {
   int* slot_off;   // [ntiles]
   int* item_tile;  // [bound on work items]
   int* n_items;    // [1]
-  float2* partial; // [bound on partial slots, kTilePx]
+  float2* partial; // [bound on partial slots, kSegTilePx]
 };
 
 inline GatherScratch carve_gather(Carver& cv, int64_t H, int64_t W, int64_t h, int64_t w, int64_t B) {
-  const TileGrid tg = tile_grid(H, W);
+  const TileGrid tg = tile_grid(h, w, B, H, W);
   const GatherBounds gb = gather_bounds(tg.count(), h, w, B);
   GatherScratch g{};
   g.lists = cv.take<int>((size_t)(tg.count() * B));
@@ -456,7 +467,7 @@ inline GatherScratch carve_gather(Carver& cv, int64_t H, int64_t W, int64_t h, i
   g.slot_off = cv.take<int>((size_t)tg.count());
   g.item_tile = cv.take<int>((size_t)gb.items);
   g.n_items = cv.take<int>(1);
-  g.partial = cv.take<float2>((size_t)(gb.slots * kTilePx));
+  g.partial = cv.take<float2>((size_t)(gb.slots * kSegTilePx));
   return g;
 }
 
@@ -466,8 +477,8 @@ inline GatherScratch carve_gather(Carver& cv, int64_t H, int64_t W, int64_t h, i
 // The list job of a gather (to run as side work of an earlier launch).
 inline TileLists gather_lists_work(const GatherScratch& g, const int64_t* pos, int64_t B, int64_t h, int64_t w,
                                    int64_t H, int64_t W, int64_t off_y, int64_t off_x) {
-  const TileGrid tg = tile_grid(H, W);
-  return TileLists{pos, B, (int)h, (int)w, H, W, off_y, off_x, tg.tiles_x, g.lists, g.counts, tg.count()};
+  const TileGrid tg = tile_grid(h, w, B, H, W);
+  return TileLists{pos, B, (int)h, (int)w, H, W, off_y, off_x, tg.tiles_x, tg.tile_h, g.lists, g.counts, tg.count()};
 }
 
 // grad (+)= the windows' contributions: tile lists (unless lists_done: an
@@ -477,7 +488,7 @@ inline ptx_status run_tile_gather(const GatherScratch& g, const float2* contrib,
                                   const int64_t* pos, int64_t B, int64_t off_y, int64_t off_x, float2* grad,
                                   int64_t H, int64_t W, int accumulate, hipStream_t s, bool lists_done = false,
                                   const SideWork& side = SideWork{}) {
-  const TileGrid tg = tile_grid(H, W);
+  const TileGrid tg = tile_grid(h, w, B, H, W);
   ptx_status st;
   if (!lists_done) {
     const TileLists tl = gather_lists_work(g, pos, B, h, w, H, W, off_y, off_x);
@@ -498,7 +509,7 @@ inline ptx_status run_tile_gather(const GatherScratch& g, const float2* contrib,
       g.item_tile, g.n_items, g.partial, grad, H, W, accumulate, side, gb.items);
   if ((st = check_launch("tile gather", s)) != PTX_OK) return st;
   if (gb.slots > 0) {
-    k_gather_combine<<<dim3((unsigned)(tg.count() * (kTilePx / 256))), dim3(256), 0, s>>>(g.counts, g.slot_off,
+    k_gather_combine<<<dim3((unsigned)(tg.count() * (kSegTilePx / 256))), dim3(256), 0, s>>>(g.counts, g.slot_off,
                                                                                           tg.tiles_x, g.partial,
                                                                                           grad, H, W);
     if ((st = check_launch("gather combine", s)) != PTX_OK) return st;

@@ -1,9 +1,9 @@
 """The deterministic object/spectrum gradient gather in both of its modes.
 
 Windows that overlap by fewer than eight views per pixel on average are
-gathered in one pass per 16 x 128 tile; denser overlaps split every tile's view
-list into segments of eight views, gathered in parallel and combined in
-segment order.  Each mode is checked against the FP64 oracle (norm-wise
+gathered in one pass per 4 x 128 tile; denser overlaps split the view list of
+every 16 x 128 tile into segments of eight views, gathered in parallel and
+combined in segment order.  Each mode is checked against the FP64 oracle (norm-wise
 relative error <= 1e-5, FP32), against the atomic scatter, for run-to-run
 bitwise equality, and with gradient accumulation (PTX_FLAG_ACCUMULATE).
 """
@@ -38,8 +38,10 @@ def _base_case(rng, H, h, B, spot):
 
 
 # (H, h, B, spot): single pass, segmented, segmented with >200-view tiles, segmented on a
-# width of partial tiles
-CASES = [(256, 64, 9, False), (128, 64, 40, False), (160, 32, 203, True), (200, 64, 80, False)]
+# width of partial tiles, single pass on a height of partial tiles, single pass with 70-view
+# lists (more than one 64-view list chunk)
+CASES = [(256, 64, 9, False), (128, 64, 40, False), (160, 32, 203, True), (200, 64, 80, False),
+         (203, 64, 12, False), (512, 64, 70, True)]
 
 
 @pytest.mark.parametrize("H,h,B,spot", CASES)
