@@ -39,14 +39,14 @@ struct FpmGatherRows {
     constexpr int T = N / E;
     const int64_t b = valid ? row / h : 0;
     const int r = valid ? (int)(row - b * h) : 0;
-    const int64_t sy = pmod(led[2 * b + 1] + r + H / 2, H);
+    const int64_t sy = (led[2 * b + 1] + r + H / 2) & (H - 1);
     const int64_t sx0 = led[2 * b] + W / 2;
     const float2* srow = spec + sy * W;
     const float2* prow = pupil + (int64_t)r * N;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int i = t + q * T;
-      v[q] = valid ? c_mul(srow[pmod(sx0 + i, W)], prow[i]) : c_zero();
+      v[q] = valid ? c_mul(srow[(sx0 + i) & (W - 1)], prow[i]) : c_zero();
     }
   }
   template <int N, int E>
@@ -237,14 +237,19 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
 }
 
 // ---------------------------------------------------------------- R2: row FFT, pupil partials, spectrum scatter
+// H and W are powers of two: the sub-spectrum wraps with masks.
 template <int N, int E>
 __global__ void __launch_bounds__(256)
     k_fpm_scatter_rows(const float2* __restrict__ tb, const float2* __restrict__ spec, int64_t H, int64_t W,
                        const float2* __restrict__ pupil, int h, const int64_t* __restrict__ led, int64_t B, int chunk,
                        int ngroups, float inv_n, float2* __restrict__ gspec, float2* __restrict__ contrib,
-                       float2* __restrict__ gq_part) {
+                       float2* __restrict__ gq_part, SideWork side, int64_t main_blocks) {
   using Cf = RowCfg<N, E>;
   constexpr int T = Cf::T, S = Cf::S;
+  if ((int64_t)blockIdx.x >= main_blocks) {  // work-groups riding along
+    side.run((int64_t)blockIdx.x - main_blocks);
+    return;
+  }
   __shared__ float2 lds[S * lds_len<N>()];
   const int seq = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -267,13 +272,13 @@ __global__ void __launch_bounds__(256)
     for (int q = 0; q < E; ++q) v[q] = rv ? src[t + q * T] : c_zero();
     fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
     if (rv) {
-      const int64_t sy = pmod(led[2 * b + 1] + r + H / 2, H);
+      const int64_t sy = (led[2 * b + 1] + r + H / 2) & (H - 1);
       const int64_t sx0 = led[2 * b] + W / 2;
 #pragma unroll
       for (int q = 0; q < E; ++q) {
         const int i = t + q * T;
         const float2 g = c_scale(v[q], inv_n);
-        const int64_t k = sy * W + pmod(sx0 + i, W);
+        const int64_t k = sy * W + ((sx0 + i) & (W - 1));
         acc[q] = c_add(acc[q], c_mulc(g, spec[k]));
         const float2 val = c_mulc(g, qc[q]);
         if (contrib) {
@@ -431,9 +436,10 @@ template <int N, int E>
 struct ScatterRun {
   static ptx_status run(const float2* tb, const float2* spec, int64_t H, int64_t W, const float2* pupil, int h,
                         const int64_t* led, int64_t B, const Plan& p, float inv_n, float2* gspec, float2* contrib,
-                        float2* gq_part, hipStream_t s) {
-    k_fpm_scatter_rows<N, E><<<dim3((unsigned)(p.ngroups * p.nchunks)), dim3(256), 0, s>>>(
-        tb, spec, H, W, pupil, h, led, B, p.chunk, p.ngroups, inv_n, gspec, contrib, gq_part);
+                        float2* gq_part, const SideWork& side, hipStream_t s) {
+    const int64_t main_blocks = (int64_t)p.ngroups * p.nchunks;
+    k_fpm_scatter_rows<N, E><<<dim3((unsigned)(main_blocks + side.blocks())), dim3(256), 0, s>>>(
+        tb, spec, H, W, pupil, h, led, B, p.chunk, p.ngroups, inv_n, gspec, contrib, gq_part, side, main_blocks);
     return check_launch("fpm scatter rows", s);
   }
 };
@@ -488,10 +494,12 @@ ptx_status spectrum_of(const float2* obj, float2* spec, int64_t H, int64_t W, hi
   return dispatch_ne<ObjFftColsOut>(PASS_COL, H, c, -1, s);
 }
 
-// R2 + pupil reduction + spectrum scatter + grad_O = HW ifft2(grad_O^)
+// R2 + pupil reduction + spectrum scatter + grad_O = HW ifft2(grad_O^).  The
+// per-LED loss sums (a step's) and the tile lists ride along with the scatter
+// launch, the pupil-partial sum with the tile gather (deterministic mode).
 ptx_status fpm_tail(const FpmWorkspace& ws, const Plan& p, const float2* spec, int64_t H, int64_t W,
                     const float2* pupil, int64_t h, int64_t w, const int64_t* led, int64_t B, float2* gobj,
-                    float2* gpupil, unsigned flags, hipStream_t s) {
+                    float2* gpupil, unsigned flags, hipStream_t s, const LossRows& loss = LossRows{}) {
   const bool det = use_segments(flags);
   const bool acc = flags & PTX_FLAG_ACCUMULATE;
   ptx_status st = PTX_OK;
@@ -500,13 +508,21 @@ ptx_status fpm_tail(const FpmWorkspace& ws, const Plan& p, const float2* spec, i
     if ((st = check_launch("zero fill", s)) != PTX_OK) return st;
   }
   const float inv_n = 1.0f / (float)(h * w);
-  st = dispatch_ne<ScatterRun>(PASS_ROW_LOOP, w, ws.tb, spec, H, W, pupil, (int)h, led, B, p, inv_n, ws.gspec, det ? ws.tb : nullptr,
-                              ws.gq_part, s);
+  SideWork with_scatter{};
+  if (det) with_scatter.lists = gather_lists_work(ws.gather, led, B, h, w, H, W, H / 2, W / 2);
+  with_scatter.loss = loss;
+  st = dispatch_ne<ScatterRun>(PASS_ROW_LOOP, w, ws.tb, spec, H, W, pupil, (int)h, led, B, p, inv_n, ws.gspec,
+                              det ? ws.tb : nullptr, ws.gq_part, with_scatter, s);
   if (st != PTX_OK) return st;
-  sum_parts2(ws.gq_part, p.nchunks, h * w, gpupil, acc ? 1 : 0, s);
-  if ((st = check_launch("pupil partial sum", s)) != PTX_OK) return st;
+  const SumParts2 pupil_sum{ws.gq_part, p.nchunks, h * w, gpupil, acc ? 1 : 0, 0};
   if (det) {
-    if ((st = run_tile_gather(ws.gather, ws.tb, h, w, led, B, H / 2, W / 2, ws.gspec, H, W, 0, s)) != PTX_OK) return st;
+    SideWork with_gather{};
+    with_gather.sums = pupil_sum;
+    st = run_tile_gather(ws.gather, ws.tb, h, w, led, B, H / 2, W / 2, ws.gspec, H, W, 0, s, true, with_gather);
+    if (st != PTX_OK) return st;
+  } else {
+    sum_parts2(ws.gq_part, p.nchunks, h * w, gpupil, acc ? 1 : 0, s);
+    if ((st = check_launch("pupil partial sum", s)) != PTX_OK) return st;
   }
   st = dispatch_ne<ObjFftRows>(PASS_ROW, W, ws.gspec, ws.gspec, H, +1, s);
   if (st != PTX_OK) return st;
@@ -634,10 +650,8 @@ ptx_status ptx_fpm_step(const void* object_field, int64_t H, int64_t W, const vo
   const float inv_n = 1.0f / (float)(h * w);
   st = dispatch_ne<FpmStepColsRun>(PASS_COL, h, ws.tb, measured, amplitude, ws.loss_part, B, w, 2.0f * loss_scale, inv_n, s);
   if (st != PTX_OK) return st;
-  loss_rows(ws.loss_part, loss_parts, B, inv_n, loss_per_position, s);
-  if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
   return fpm_tail(ws, p, ws.spec, H, W, (const float2*)pupil, h, w, led, B, (float2*)grad_object,
-                  (float2*)grad_pupil, flags, s);
+                  (float2*)grad_pupil, flags, s, LossRows{ws.loss_part, loss_parts, inv_n, loss_per_position, B});
 }
 
 }  // extern "C"
