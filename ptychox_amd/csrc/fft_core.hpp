@@ -187,10 +187,12 @@ __device__ __forceinline__ void lds_barrier() {
 
 // LDS_TW: the stage's base twiddles come from `lds_tw`, an LDS copy of
 // w_N^k (k < N, forward direction; see load_lds_twiddles) instead of the
-// global table.
-template <int N, int E, int DIR, int R, int NS, bool LAST, bool SYNC_FIRST, bool LDS_TW = false>
+// global table.  REG_TW: every twiddle of the stage comes from `reg_tw`
+// (RegTwiddles below, forward direction, conjugated here for the inverse).
+template <int N, int E, int DIR, int R, int NS, bool LAST, bool SYNC_FIRST, bool LDS_TW = false, bool REG_TW = false>
 __device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restrict__ lds,
-                                          const float2* __restrict__ lds_tw = nullptr) {
+                                          const float2* __restrict__ lds_tw = nullptr,
+                                          const float2* reg_tw = nullptr) {
   constexpr int T = N / E;
   constexpr int NB = E / R;               // butterflies per thread
   constexpr int TWS = TW_MAX / (NS * R);  // twiddle stride into the table
@@ -203,7 +205,10 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restr
     float2 a[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) a[m] = v[i + m * NB];
-    if constexpr (NS > 1) {
+    if constexpr (NS > 1 && REG_TW) {
+#pragma unroll
+      for (int m = 1; m < R; ++m) a[m] = DIR > 0 ? c_mulc(a[m], reg_tw[i + m * NB]) : c_mul(a[m], reg_tw[i + m * NB]);
+    } else if constexpr (NS > 1) {
       // w^(k m) by recurrence from one table load: relative error grows by
       // ~1 ulp per step (< 1e-6 for R = 16) and saves R - 2 memory loads
       float2 w1;
@@ -271,6 +276,81 @@ __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, float2* lds0, 
     fft_stages<N, E, DIR, PINGPONG, S + 1, LDS_TW>(v, t, lds0, lds1, lds_tw);
   }
 }
+
+// Stage twiddles held in registers, for passes whose threads keep the same
+// element indices from one sequence to the next (a column pass: thread t holds
+// rows t + q T of every tile).  Stage S >= 1 multiplies input m of butterfly i
+// by w^(k m), k = (t + i T) mod NS; the table values are loaded once per
+// thread and serve every sequence, forward and (conjugated) inverse, in place
+// of the per-stage recurrence.  (E - E/R) float2 per stage.
+template <int N, int E>
+struct RegTwiddles {
+  using Sh = FftShape<N, E>;
+  static constexpr int NST = Sh::STAGES > 1 ? Sh::STAGES - 1 : 1;
+  float2 w[NST][E];  // [S - 1][i + m NB], forward direction; entries m = 0 unused
+  template <int S = 1>
+  __device__ __forceinline__ void init(int t) {
+    if constexpr (S < Sh::STAGES) {
+      constexpr int T = N / E;
+      constexpr int R = S < Sh::FULL ? E : Sh::REM;
+      constexpr int NS = ipow(E, S);
+      constexpr int NB = E / R;
+      constexpr int TWS = TW_MAX / (NS * R);
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int k = (t + i * T) & (NS - 1);
+        w[S - 1][i] = make_float2(1.0f, 0.0f);
+#pragma unroll
+        for (int m = 1; m < R; ++m) w[S - 1][i + m * NB] = twiddle<-1>(k * m * TWS);
+      }
+      init<S + 1>(t);
+    }
+  }
+};
+
+template <int N, int E, int DIR, bool PINGPONG, int S = 0>
+__device__ __forceinline__ void fft_stages_reg(float2 (&v)[E], int t, float2* lds0, float2* lds1,
+                                               const RegTwiddles<N, E>& tw) {
+  using Sh = FftShape<N, E>;
+  if constexpr (S < Sh::STAGES) {
+    constexpr int R = S < Sh::FULL ? E : Sh::REM;
+    constexpr int NS = ipow(E, S);
+    constexpr bool LAST = S == Sh::STAGES - 1;
+    constexpr bool SYNC_FIRST = !PINGPONG && S > 0 && !LAST;
+    fft_stage<N, E, DIR, R, NS, LAST, SYNC_FIRST, false, true>(v, t, (PINGPONG && (S & 1)) ? lds1 : lds0, nullptr,
+                                                               tw.w[S > 0 ? S - 1 : 0]);
+    fft_stages_reg<N, E, DIR, PINGPONG, S + 1>(v, t, lds0, lds1, tw);
+  }
+}
+
+// Stage twiddles of a kernel that transforms many sequences with a fixed
+// thread-to-index map: in registers at up to 8 elements per thread (up to 14
+// float2 per stage) unless the kernel opts out (ALLOW_REG = false: its own
+// state already fills the register budget), otherwise from an LDS table.  The
+// kernel declares `__shared__ float2 s_tw[LoopTwiddles<...>::TABLE]` and calls
+// init uniformly before its first transform.
+template <int N, int E, bool ALLOW_REG = true>
+struct LoopTwiddles {
+  static constexpr bool REG = ALLOW_REG && E <= 8;
+  static constexpr int TABLE = REG ? 1 : N;
+  RegTwiddles<N, E> reg;
+  const float2* table;
+  __device__ __forceinline__ void init(int t, float2* s_tw) {
+    if constexpr (REG) {
+      reg.init(t);
+    } else {
+      load_lds_twiddles<N>(s_tw);
+      table = s_tw;
+    }
+  }
+  template <int DIR, bool PINGPONG = true>
+  __device__ __forceinline__ void stages(float2 (&v)[E], int t, float2* lds0, float2* lds1) const {
+    if constexpr (REG)
+      fft_stages_reg<N, E, DIR, PINGPONG>(v, t, lds0, lds1, reg);
+    else
+      fft_stages<N, E, DIR, PINGPONG, 0, true>(v, t, lds0, lds1, table);
+  }
+};
 
 // Transform one sequence held in natural layout.  Every thread of the work-group
 // must call this uniformly (it contains work-group barriers when N > E).

@@ -153,12 +153,14 @@ inline void launch_cols(const Op& op, int64_t frames, int64_t width, hipStream_t
 // regions with one barrier each; the inverse transform starts on the region
 // after the forward's last, so consecutive exchanges never share a region and
 // no other barrier is needed.
-template <int N, int E, int C>
+// REG_TW: the kernel holds its stage twiddles in LoopTwiddles<N, E, REG_TW>
+template <int N, int E, int C, bool REG_TW = true>
 struct PipeColsShape {
   static constexpr int T = ColCfg<N, E>::T;
   static constexpr int THREADS = T * C;
   static constexpr int EXCHANGES = FftShape<N, E>::STAGES - 1;
-  static constexpr size_t LDS_BYTES = (2 * (size_t)C * col_stride<N, C>() + N) * sizeof(float2);
+  static constexpr size_t LDS_BYTES =
+      (2 * (size_t)C * col_stride<N, C>() + LoopTwiddles<N, E, REG_TW>::TABLE) * sizeof(float2);
   static_assert(LDS_BYTES <= 160 * 1024, "LDS budget");
   // resident work-groups per CU: threads and LDS
   static constexpr int PER_CU = (1024 / THREADS) < (int)((160 * 1024) / LDS_BYTES) ? (1024 / THREADS)
@@ -173,7 +175,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   constexpr int T = Sh::T;
   __shared__ float2 lds0[C * col_stride<N, C>()];
   __shared__ float2 lds1[C * col_stride<N, C>()];
-  __shared__ float2 s_tw[N];
+  __shared__ float2 s_tw[LoopTwiddles<N, E>::TABLE];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
   const int64_t fsz = (int64_t)N * width;               // elements per frame
@@ -185,7 +187,8 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   const TileIdx ti = split_tile(xcd_paired_block(blockIdx.x, gridDim.x), tiles_per_frame);
   const uint32_t col0 = (uint32_t)(ti.tile * C);
   const unsigned group = (unsigned)ti.frame;  // < ngroups <= frames
-  load_lds_twiddles<N>(s_tw);
+  LoopTwiddles<N, E> tw;  // thread t holds the same rows of every frame
+  tw.init(t, s_tw);
   float2 tr[E];
   {
     const auto rtr = buffer_rsrc(transfer, fbytes);
@@ -207,13 +210,13 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     const int tt = (int)(tid / C), cc = (int)(tid % C);
     float2* l0 = &lds0[cc * col_stride<N, C>()];
     float2* l1 = &lds1[cc * col_stride<N, C>()];
-    fft_stages<N, E, -1, true, 0, true>(v, tt, l0, l1, s_tw);
+    tw.template stages<-1>(v, tt, l0, l1);
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = CONJ ? c_mulc(v[q], tr[q]) : c_mul(v[q], tr[q]);
     if constexpr (Sh::EXCHANGES % 2 == 1)
-      fft_stages<N, E, +1, true, 0, true>(v, tt, l1, l0, s_tw);
+      tw.template stages<+1>(v, tt, l1, l0);
     else
-      fft_stages<N, E, +1, true, 0, true>(v, tt, l0, l1, s_tw);
+      tw.template stages<+1>(v, tt, l0, l1);
     const auto rt = buffer_rsrc(tb + (int64_t)f * fsz, fbytes);
 #pragma unroll
     for (int q = 0; q < E; ++q) buf_store_f2(v[q], rt, lane * 8u, (q * rstep + col0) * 8u);

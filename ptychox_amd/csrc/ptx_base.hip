@@ -93,7 +93,8 @@ struct StepColsShape {
   static constexpr int THREADS = T * C;
   static constexpr int WAVES = (THREADS + 63) / 64;
   static constexpr int EXCHANGES = FftShape<N, E>::STAGES - 1;
-  static constexpr size_t LDS_BYTES = (2 * (size_t)C * col_stride<N, C>() + N) * sizeof(float2);
+  static constexpr size_t LDS_BYTES =
+      (2 * (size_t)C * col_stride<N, C>() + LoopTwiddles<N, E>::TABLE) * sizeof(float2);
 };
 
 template <int N, int E, int C, bool AMP>
@@ -106,7 +107,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   static_assert(Sh::LDS_BYTES <= 160 * 1024, "LDS budget");
   __shared__ float2 lds0[C * col_stride<N, C>()];
   __shared__ float2 lds1[C * col_stride<N, C>()];
-  __shared__ float2 s_tw[N];
+  __shared__ float2 s_tw[LoopTwiddles<N, E>::TABLE];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
   const int wave = threadIdx.x >> 6;
@@ -116,7 +117,8 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   const uint32_t lane = (uint32_t)(t * (int)width + c);  // element offset inside a tile (row t, column c)
   const uint32_t rstep = (uint32_t)(T * width);          // elements from row t + q T to t + (q + 1) T
   const unsigned nt = (unsigned)ntiles;
-  load_lds_twiddles<N>(s_tw);
+  LoopTwiddles<N, E> tw;  // thread t holds the same rows of every tile
+  tw.init(t, s_tw);
   // loads of tile kk into (dv, dm); the measured frame of a view is the
   // batch's own (order == null) or measured[order[frame]].  A tile past the
   // end gets zero-length buffers: its loads are issued but return zero
@@ -152,7 +154,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     asm volatile("" : "+v"(tt), "+v"(cc));
     float2* l0 = &lds0[cc * col_stride<N, C>()];
     float2* l1 = &lds1[cc * col_stride<N, C>()];
-    fft_stages<N, E, -1, true, 0, true>(v, tt, l0, l1, s_tw);
+    tw.template stages<-1>(v, tt, l0, l1);
     if constexpr (Sh::EXCHANGES % 2 == 1) lds_barrier();
     float lsum = 0.0f;
 #pragma unroll
@@ -176,7 +178,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
       if (off < Sh::THREADS) lsum += __shfl_xor(lsum, off, 64);
     buf_store_f1(lsum, buffer_rsrc(loss_part + (ti.frame * tiles_per_frame + ti.tile) * Sh::WAVES + wave, 4u),
                  lane0 ? 0u : 64u, 0u);
-    fft_stages<N, E, +1, true, 0, true>(v, tt, l0, l1, s_tw);
+    tw.template stages<+1>(v, tt, l0, l1);
     if constexpr (Sh::EXCHANGES % 2 == 1) lds_barrier();
     const auto rt = buffer_rsrc(tb + ti.frame * fsz, fbytes);
 #pragma unroll

@@ -351,15 +351,16 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     k_coded_spec_cols(const float2* __restrict__ tb, const float2* __restrict__ h1, const float2* __restrict__ ex,
                       const float2* __restrict__ ey, int64_t B, int64_t width, int64_t tiles, int chunk, float inv_n,
                       float2* __restrict__ gs_part) {
-  using Sh = PipeColsShape<N, E, C>;
+  using Sh = PipeColsShape<N, E, C, false>;
   constexpr int T = Sh::T;
   constexpr bool ODD = Sh::EXCHANGES % 2 == 1;
   __shared__ float2 lds0[C * col_stride<N, C>()];
   __shared__ float2 lds1[C * col_stride<N, C>()];
-  __shared__ float2 s_tw[N];
-  load_lds_twiddles<N>(s_tw);
+  __shared__ float2 s_tw[LoopTwiddles<N, E, false>::TABLE];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
+  LoopTwiddles<N, E, false> tw;  // the accumulator fills the register budget: LDS table
+  tw.init(t, s_tw);
   const TileIdx ti = split_tile(xcd_paired_block(blockIdx.x, gridDim.x), tiles);
   const int64_t tile = ti.tile;
   const unsigned ch = (unsigned)ti.frame;  // chunk
@@ -394,9 +395,9 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     float2* l0 = &lds0[cc * col_stride<N, C>()];
     float2* l1 = &lds1[cc * col_stride<N, C>()];
     if (second)
-      fft_stages<N, E, -1, true, 0, true>(v, tt, l1, l0, s_tw);
+      tw.template stages<-1>(v, tt, l1, l0);
     else
-      fft_stages<N, E, -1, true, 0, true>(v, tt, l0, l1, s_tw);
+      tw.template stages<-1>(v, tt, l0, l1);
 #pragma unroll
     for (int q = 0; q < E; ++q) acc[q] = c_add(acc[q], c_mulc(v[q], c_mul(ry[q], rx)));  // v conj(R_b)
   };
@@ -438,10 +439,11 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   constexpr bool ODD = Sh::EXCHANGES % 2 == 1;
   __shared__ float2 lds0[C * col_stride<N, C>()];
   __shared__ float2 lds1[C * col_stride<N, C>()];
-  __shared__ float2 s_tw[N];
-  load_lds_twiddles<N>(s_tw);
+  __shared__ float2 s_tw[LoopTwiddles<N, E>::TABLE];
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
+  LoopTwiddles<N, E> tw;  // thread t holds the same rows of every view
+  tw.init(t, s_tw);
   const TileIdx ti = split_tile(xcd_paired_block(blockIdx.x, gridDim.x), tiles);
   const int64_t col = ti.tile * C + c;
   const unsigned b0 = (unsigned)ti.frame * (unsigned)chunk;  // < B
@@ -475,9 +477,9 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = c_mul(base[q], c_mul(ry[q], rx));
     if (second)
-      fft_stages<N, E, +1, true, 0, true>(v, tt, l1, l0, s_tw);
+      tw.template stages<+1>(v, tt, l1, l0);
     else
-      fft_stages<N, E, +1, true, 0, true>(v, tt, l0, l1, s_tw);
+      tw.template stages<+1>(v, tt, l0, l1);
     const auto r = buffer_rsrc(out + (int64_t)b * N * width, fbytes);
 #pragma unroll
     for (int q = 0; q < E; ++q) buf_store_f2(v[q], r, lane, q * rstep + col0);
