@@ -37,7 +37,7 @@ OBJ, PROBE, BATCH = 2048, 512, 64
 # (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a plain 256 MiB copy kernel
 # reads back; profiles/base_step_obj2048_probe512_b64_pmc_traffic.md).  The
 # counters need their own profiled runs, so the figure is recorded here.
-MEASURED_TRAFFIC = {"base step cols": 336_070_656}
+MEASURED_TRAFFIC = {"base step cols": 336_001_600}
 
 # Stated latency bound of the served mix: p99 from arrival to completion at
 # every measured Poisson rate up to 0.9x the server's saturation throughput.
