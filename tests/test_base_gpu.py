@@ -91,6 +91,35 @@ def test_base_native_vs_torch_path_full_size(gpu, det):
     assert max(e) < TOL
 
 
+@pytest.mark.parametrize("scatter", ["segments", "atomic"])
+def test_base_fused_step_full_size_vs_torch_path(gpu, scatter):
+    """The fused step at the headline shapes (object 2048^2, probe 512^2, the
+    first 64 positions of the 30 x 30 raster: a band of the object, as the
+    benchmark's mini-batches) against the reference torch path with
+    F.mse_loss: per-position losses and both gradients."""
+    from ptychox_amd import ptychoX_base
+    from ptychox_amd.base import BaseStep
+
+    rng = np.random.default_rng(6)
+    s = to_dev(smooth_object(rng, 2048, 4.0), gpu)
+    p = to_dev(disc_probe(512, 128), gpu)
+    pos = to_dev(raster_positions(rng, 30, 52, 14, 3)[:64], gpu)
+    with torch.no_grad():
+        y = ptychoX_base(s * 1.05, p, pos, use_cuda_kernels=False).contiguous()
+    st, pt = s.clone().requires_grad_(True), p.clone().requires_grad_(True)
+    amp = ptychoX_base(st, pt, pos, use_cuda_kernels=False)
+    torch.nn.functional.mse_loss(amp, y).backward()
+    per = ((amp.detach() - y) ** 2).mean(dim=(1, 2))
+    step = BaseStep(s.shape, p.shape, 64, gpu, scatter=scatter)
+    gs, gp = torch.empty_like(s), torch.empty_like(p)
+    lb = step(s, p, pos, y, gs, gp)
+    torch.cuda.synchronize()
+    e = (nrel(lb, per), nrel(gs, st.grad), nrel(gp, pt.grad))
+    print(f"base fused step 2048/512/B64 {scatter}: loss_b {e[0]:.2e} grad_sample {e[1]:.2e} "
+          f"grad_probe {e[2]:.2e} (vs torch path)", flush=True)
+    assert max(e) < TOL
+
+
 def test_base_deterministic_bitwise(gpu):
     from ptychox_amd import ptychoX_base, set_scatter_mode
 
