@@ -49,6 +49,7 @@ def test_fpm_vs_oracle(gpu, H, h, B, det):
     n_side = int(round(np.sqrt(B)))
     led = _leds(n_side, max(h // 8, 1), (H - h) // 2, H)[:B]
     led[0] = [-3, H - 2]  # wrap-around crop
+    led[1] = [-H - 5, -H // 2 - 9]  # below zero, more than one period out
     g = rng.standard_normal((B, h, h)).astype(np.float32)
     ot = to_dev(o, gpu).requires_grad_(True)
     qt = to_dev(q, gpu).requires_grad_(True)
@@ -115,6 +116,8 @@ def test_fpm_step_vs_oracle(gpu, det):
     o = _rough_object(rng, 128)
     q = _pupil(32, 12, rng)
     led = _leds(5, 6, 48, 128)
+    led[0] = [-128 - 70, 3 * 128 + 5]  # crops wrapping several periods out, both signs
+    led[1] = [-70, -200]
     y = po.fpm_forward(o * 1.1, q, led)["amplitude"].astype(np.float32)
     loss, per, go, gq = po.fpm_train_step(o, q, led, y)
     step = FpmStep(o.shape, q.shape, 25, gpu, scatter="segments" if det else "atomic")
