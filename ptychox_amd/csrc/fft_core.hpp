@@ -45,6 +45,19 @@ __device__ __forceinline__ float2 c_zero() { return make_float2(0.0f, 0.0f); }
 // divides of the loss / phase epilogues: the IEEE divide sequence costs ~10
 // instructions and several registers per element.  x >= FLT_EPSILON there.
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// |a| and 1/max(|a|, floor) from one hardware reciprocal square root
+// (v_rsq_f32) instead of a square root and a reciprocal (two quarter-rate
+// instructions): with s = |a|^2 and r = rsq(max(s, FLT_MIN)), |a| = s r and
+// 1/max(|a|, floor) = min(r, 1/floor).  Above |a| = 1.1e-19 (s = FLT_MIN) the
+// magnitude is within ~2 ulp; below, it is off by less than 1.1e-19 absolute.
+struct AbsInv {
+  float a, inv;
+};
+__device__ __forceinline__ AbsInv c_abs_inv(float2 v, float inv_floor) {
+  const float s = v.x * v.x + v.y * v.y;
+  const float r = __builtin_amdgcn_rsqf(fmaxf(s, 1.17549435e-38f));
+  return AbsInv{s * r, fminf(r, inv_floor)};
+}
 
 // ---------------------------------------------------------------- compile-time twiddle table
 

@@ -159,11 +159,11 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     float lsum = 0.0f;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      const float a = c_abs(v[q]);
-      const float d = a - m[q];
+      const AbsInv ai = c_abs_inv(v[q], 1.0f / kFltEps);
+      const float d = ai.a - m[q];
       lsum += d * d;
-      v[q] = c_scale(v[q], gscale * d * fast_rcp(fmaxf(a, kFltEps)));
-      m[q] = a;
+      v[q] = c_scale(v[q], gscale * d * ai.inv);
+      m[q] = ai.a;
     }
     // amplitudes (AMP: requested) and the wave's loss partial (lane 0; the
     // other lanes' offsets fall outside the 4-byte buffer): stores without

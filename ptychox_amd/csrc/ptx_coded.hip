@@ -239,11 +239,12 @@ __global__ void __launch_bounds__(256)
   for (int q = 0; q < E; ++q) {
     const int64_t k = row * N + t + q * T;
     const float2 e = c_scale(v[q], inv_n);
-    const float a = c_abs(e);
+    const AbsInv ai = c_abs_inv(e, 1.0f / kCodedAmpEps);
+    const float a = ai.a;
     const float d = valid ? a - measured[k] : 0.0f;
     lsum += d * d;
     if (valid && amp_out) amp_out[k] = a;
-    v[q] = c_scale(e, gscale * d * fast_rcp(fmaxf(a, kCodedAmpEps)));
+    v[q] = c_scale(e, gscale * d * ai.inv);
   }
   red[threadIdx.x] = lsum;
   __syncthreads();

@@ -219,11 +219,13 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int64_t k = off + ((t + q * T) * (int)width + (int)col);
-    const float a = c_abs(v[q]) * inv_n;
+    // a = inv_n |v|; inv_n / max(a, eps) = 1 / max(|v|, eps / inv_n)
+    const AbsInv ai = c_abs_inv(v[q], inv_n / kFltEps);
+    const float a = ai.a * inv_n;
     const float d = a - measured[k];
     lsum += d * d;
     if (amp_out) amp_out[k] = a;
-    v[q] = c_scale(v[q], gscale * d * inv_n * fast_rcp(fmaxf(a, kFltEps)));
+    v[q] = c_scale(v[q], gscale * d * ai.inv);
   }
   // one loss partial per wave (no work-group barrier)
 #pragma unroll
