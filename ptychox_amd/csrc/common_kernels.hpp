@@ -222,10 +222,11 @@ constexpr int kSegTilePx = kSegTileH * kTileW;
 
 // Work items and partial-tile slots a gather can need: every tile has at
 // least one segment, and the segments beyond the first are bounded by the
-// list entries divided by kSegViews.  A view's window meets at most
-// h/16 + 2 tile rows of a segmented gather (wrap-around on a grid of whole tiles included) and the
-// list test can add the grid's last, partial tile row; likewise for columns,
-// hence (h/16 + 3) x (w/128 + 3) list entries per view.
+// list entries divided by kSegViews.  A view's window meets at most h/16 + 2
+// rows of the segmented gather's 16-row tiles (wrap-around on a grid of whole
+// tiles included) and the list test can add the grid's last, partial tile
+// row; likewise for columns, hence (h/16 + 3) x (w/128 + 3) list entries per
+// view.
 struct GatherBounds {
   int64_t items, slots;
 };
