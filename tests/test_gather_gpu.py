@@ -70,6 +70,30 @@ def test_base_step_gather_modes_vs_oracle(gpu, H, h, B, spot):
     assert nrel(out["segments"], out["atomic"]) < TOL
 
 
+# rectangular objects with partial tiles along both axes: single pass (203 x 330) and segmented
+# (100 x 72)
+@pytest.mark.parametrize("H,W,h,B", [(203, 330, 64, 12), (100, 72, 64, 60)])
+def test_base_step_gather_rectangular(gpu, H, W, h, B):
+    from ptychox_amd.base import BaseStep
+
+    rng = np.random.default_rng(H + W + B)
+    s = (smooth_object(rng, max(H, W))[:H, :W] + 0.05 * crandn(rng, H, W)).astype(np.complex64)
+    p = disc_probe(h, h / 4)
+    pos = np.stack([rng.integers(0, W - h + 1, B), rng.integers(0, H - h + 1, B)], axis=1).astype(np.int64)
+    y = np.abs(po.base_forward(s * 1.05, p, pos)["amplitude"]).astype(np.float32)
+    _, per, gs, gp = po.base_train_step(s, p, pos, y)
+    for scatter in ("segments", "atomic"):
+        step = BaseStep(s.shape, p.shape, B, gpu, scatter=scatter)
+        gst = torch.empty(s.shape, dtype=torch.complex64, device=gpu)
+        gpt = torch.empty(p.shape, dtype=torch.complex64, device=gpu)
+        lb = step(to_dev(s, gpu), to_dev(p, gpu), to_dev(pos, gpu), to_dev(y, gpu), gst, gpt)
+        torch.cuda.synchronize()
+        e = (nrel(lb, per), nrel(gst, gs), nrel(gpt, gp))
+        print(f"base gather {H}x{W}/{h}/B{B} segmented={B * h * h >= 8 * H * W} {scatter}: loss_b {e[0]:.2e} "
+              f"grad {e[1]:.2e} probe {e[2]:.2e} (vs FP64 oracle)", flush=True)
+        assert max(e) < TOL
+
+
 @pytest.mark.parametrize("H,h,B,spot", CASES[1:3])
 def test_base_segmented_gather_bitwise(gpu, H, h, B, spot):
     from ptychox_amd.base import BaseStep
