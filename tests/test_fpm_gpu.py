@@ -224,3 +224,40 @@ def test_fpm_full_size_central_differences(gpu):
     # error (agreement to 1e-5 at this size is tested on a noise-floor object).
     for mine, ref in zip(e_hip + f_hip, e_t32 + f_t32):
         assert mine < max(1e-4, 2.0 * ref)
+
+
+@pytest.mark.parametrize("det", [False, True])
+def test_fpm_rectangular_vs_oracle(gpu, det):
+    """Spectrum 128 x 256 and pupil 32 x 64 (rows x columns), LED crops wrapping on both axes: the
+    operator through autograd and the fused step."""
+    from ptychox_amd import ptychoX_fourier, set_scatter_mode
+    from ptychox_amd.fpm import FpmStep
+
+    rng = np.random.default_rng(21 + det)
+    H, W, h, w, B = 128, 256, 32, 64, 9
+    o = (smooth_object(rng, W)[:H, :] + 0.05 * crandn(rng, H, W)).astype(np.complex64)
+    q = (rng.uniform(0.5, 1.0, (h, w)) * np.exp(1j * rng.uniform(-1, 1, (h, w)))).astype(np.complex64)
+    led = np.stack([rng.integers(-W, W, B), rng.integers(-H, H, B)], axis=1).astype(np.int64)
+    g = rng.standard_normal((B, h, w)).astype(np.float32)
+    ot = to_dev(o, gpu).requires_grad_(True)
+    qt = to_dev(q, gpu).requires_grad_(True)
+    set_scatter_mode("segments" if det else "atomic")
+    try:
+        amp = ptychoX_fourier(ot, qt, to_dev(led, gpu), use_cuda_kernels=True)
+        amp.backward(to_dev(g, gpu))
+    finally:
+        set_scatter_mode(None)
+    fwd = po.fpm_forward(o, q, led)
+    go, gq = po.fpm_backward(g, o.shape, q, fwd["sub_spectrum"], fwd["image_field"], led)
+    e = (nrel(amp, fwd["amplitude"]), nrel(ot.grad, go), nrel(qt.grad, gq))
+    y = po.fpm_forward(o * 1.1, q, led)["amplitude"].astype(np.float32)
+    _, per, gos, gqs = po.fpm_train_step(o, q, led, y)
+    step = FpmStep(o.shape, q.shape, B, gpu, scatter="segments" if det else "atomic")
+    got = torch.empty(o.shape, dtype=torch.complex64, device=gpu)
+    gqt = torch.empty(q.shape, dtype=torch.complex64, device=gpu)
+    lb = step(to_dev(o, gpu), to_dev(q, gpu), to_dev(led, gpu), to_dev(y, gpu), got, gqt)
+    torch.cuda.synchronize()
+    e += (nrel(lb, per), nrel(got, gos), nrel(gqt, gqs))
+    print(f"fpm 128x256/32x64/B9 det={det}: amp {e[0]:.2e} grad_object {e[1]:.2e} grad_pupil {e[2]:.2e}; "
+          f"step loss_b {e[3]:.2e} grad_object {e[4]:.2e} grad_pupil {e[5]:.2e} (vs FP64 oracle)", flush=True)
+    assert max(e) < TOL

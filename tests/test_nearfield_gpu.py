@@ -99,3 +99,34 @@ def test_nearfield_step_vs_oracle(gpu, det):
     e = (nrel(lb, per), nrel(gst, gs), nrel(gpt, gp))
     print(f"nearfield step det={det}: loss_b {e[0]:.2e} grad_sample {e[1]:.2e} grad_probe {e[2]:.2e}", flush=True)
     assert max(e) < TOL
+
+
+@pytest.mark.parametrize("det", [False, True])
+def test_nearfield_rectangular_vs_oracle(gpu, det):
+    """Sample 160 x 224 and a 64 x 32 probe with its own 64 x 32 transfer (rows x columns)."""
+    from ptychox_amd import ptychoX_nearfield, set_scatter_mode
+
+    rng = np.random.default_rng(31 + det)
+    H, W, h, w, B = 160, 224, 64, 32, 6
+    s = smooth_object(rng, W)[:H, :]
+    yy, xx = np.mgrid[0:h, 0:w]
+    inside = ((yy - h / 2) / (h / 2)) ** 2 + ((xx - w / 2) / (w / 2)) ** 2 <= 1
+    p = (inside * np.exp(1j * rng.uniform(0, 2 * np.pi, (h, w)))).astype(np.complex64)
+    fy, fx = np.fft.fftfreq(h), np.fft.fftfreq(w)
+    t = np.exp(1j * np.pi * 60.0 * (fy[:, None] ** 2 + fx[None, :] ** 2)).astype(np.complex64)
+    pos = np.stack([rng.integers(0, W - w + 1, B), rng.integers(0, H - h + 1, B)], axis=1).astype(np.int64)
+    g = rng.standard_normal((B, h, w)).astype(np.float32)
+    st = to_dev(s, gpu).requires_grad_(True)
+    pt = to_dev(p, gpu).requires_grad_(True)
+    set_scatter_mode("segments" if det else "atomic")
+    try:
+        amp = ptychoX_nearfield(st, pt, to_dev(t, gpu), to_dev(pos, gpu), use_cuda_kernels=True)
+        amp.backward(to_dev(g, gpu))
+    finally:
+        set_scatter_mode(None)
+    fwd = po.nearfield_forward(s, p, t, pos)
+    gs, gp = po.nearfield_backward(g, s.shape, p, t, fwd["patches"], fwd["field"], pos)
+    e = (nrel(amp, fwd["amplitude"]), nrel(st.grad, gs), nrel(pt.grad, gp))
+    print(f"nearfield 160x224/64x32/B6 det={det}: amp {e[0]:.2e} grad_sample {e[1]:.2e} grad_probe {e[2]:.2e} "
+          "(vs FP64 oracle)", flush=True)
+    assert max(e) < TOL
