@@ -39,9 +39,9 @@ def _base_case(rng, H, h, B, spot):
 
 # (H, h, B, spot): single pass, segmented, segmented with >200-view tiles, segmented on a
 # width of partial tiles, single pass on a height of partial tiles, single pass with 70-view
-# lists (more than one 64-view list chunk)
+# lists (more than one 64-view list chunk), a batch of one view
 CASES = [(256, 64, 9, False), (128, 64, 40, False), (160, 32, 203, True), (200, 64, 80, False),
-         (203, 64, 12, False), (512, 64, 70, True)]
+         (203, 64, 12, False), (512, 64, 70, True), (96, 32, 1, False)]
 
 
 @pytest.mark.parametrize("H,h,B,spot", CASES)
