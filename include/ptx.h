@@ -128,6 +128,10 @@ ptx_status ptx_fft2d(void* data, int64_t batch, int64_t h, int64_t w, int invers
  * bound at base_extension.cpp:21-32) and the torch branch base/ops.py:60-66.
  * The reference also returns the patch stack; this library re-gathers patches
  * from ``sample`` in the backward instead of storing them.
+ * Positions are device data and are not checked here: a window that leaves the
+ * sample gives a zero frame (and no gradient in the adjoint and step entry
+ * points) without reading or writing outside the sample.  The Python operator
+ * checks them on the host first and raises ValueError.
  * far_field: [batch, h, w] complex64 out; amplitude: [batch, h, w] float out. */
 ptx_status ptx_base_forward(const void* sample, int64_t sample_h, int64_t sample_w,
                             const void* probe, int64_t probe_h, int64_t probe_w,
