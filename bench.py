@@ -117,9 +117,9 @@ def _timed(fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def measure_epoch(ds, dev) -> dict:
-    """Whole training epochs of the reference driver's loop (permutation,
-    900 positions in mini-batches of 64 with a partial last batch, MSE loss,
+def measure_epoch(ds, dev, batch: int = BATCH, label: str = "900 positions") -> dict:
+    """Whole training epochs of the reference driver's loop (permutation, all
+    positions in mini-batches of `batch` with a partial last batch, MSE loss,
     Adam on object and probe): the device-resident loop captured as a hipGraph,
     the same loop launched eagerly, and the plain torch loop."""
     import torch
@@ -130,10 +130,10 @@ def measure_epoch(ds, dev) -> dict:
     P = ds.positions.shape[0]
     init_s = torch.ones_like(ds.sample)
     init_p = (ds.probe.abs() > 0).to(torch.complex64)
-    out = {"positions": P, "mini_batch": BATCH}
+    out = {"positions": P, "mini_batch": batch}
     for name, graph in (("graph", True), ("eager", False)):
         gen = torch.Generator(device=dev).manual_seed(0)
-        rec = ConventionalReconstruction(init_s, init_p, ds.positions, ds.measured, BATCH, lr=1e-2, epochs=10,
+        rec = ConventionalReconstruction(init_s, init_p, ds.positions, ds.measured, batch, lr=1e-2, epochs=10,
                                          generator=gen, graph=graph)
         rec.run_epoch(1)
         torch.cuda.synchronize()
@@ -153,8 +153,8 @@ def measure_epoch(ds, dev) -> dict:
 
     def torch_epoch():
         order = torch.randperm(P, device=dev)
-        for start in range(0, P, BATCH):
-            bi = order[start : start + BATCH]
+        for start in range(0, P, batch):
+            bi = order[start : start + batch]
             loss = torch.nn.functional.mse_loss(ptychoX_base(s, p, ds.positions[bi]), ds.measured[bi])
             opt.zero_grad(set_to_none=True)
             loss.backward()
@@ -171,7 +171,7 @@ def measure_epoch(ds, dev) -> dict:
     ms = e0.elapsed_time(e1)
     out["plain_torch"] = {"ms_per_epoch": round(ms, 3), "patterns_per_s": round(P / (ms / 1e3), 1)}
     out["graph_speedup_vs_plain"] = round(ms / out["graph"]["ms_per_epoch"], 2)
-    print(f"[bench] epoch (900 positions, Adam): graph {out['graph']['ms_per_epoch']} ms, eager "
+    print(f"[bench] epoch ({label}, Adam): graph {out['graph']['ms_per_epoch']} ms, eager "
           f"{out['eager']['ms_per_epoch']} ms, plain torch {ms:.3f} ms", file=sys.stderr, flush=True)
     return out
 
@@ -504,7 +504,16 @@ def main(argv=None) -> int:
         torch.cuda.synchronize()
         del a, b
 
-    epoch = None if args.no_epoch else measure_epoch(ds, dev)
+    epoch = epoch_small = None
+    if not args.no_epoch:
+        epoch = measure_epoch(ds, dev)
+        # the reference's CPU-runnable case (object 512^2, probe 128^2, 225 positions, mini-batch 32):
+        # kernels of a few microseconds, where launches rather than HBM bound an eager epoch
+        from ptychox_amd.synthetic import conventional_dataset as _cds
+
+        small = _cds(512, 128, 32.0, 15, 24, 24, 2, 1e4, 4.0, dev, seed=0)
+        epoch_small = measure_epoch(small, dev, batch=32, label="512^2 object, 128^2 probe, 225 positions")
+        del small
     modalities = None if args.no_modalities else measure_modalities(dev, args)
     serving = None if args.no_serve else measure_serving(dev, args)
 
@@ -536,6 +545,7 @@ def main(argv=None) -> int:
         "plain_path": plain,
         "cpu_baseline": cpu,
         "epoch": epoch,
+        "epoch_small": epoch_small,
         "other_workloads": modalities,
         "serving": serving,
     }
