@@ -19,8 +19,13 @@
  *    [PTX_MIN_N, PTX_MAX_N]; other sizes return PTX_ERR_UNSUPPORTED.
  *  - a non-zero status leaves a message for ptx_last_error() (thread-local).
  *  - outputs are overwritten unless PTX_FLAG_ACCUMULATE is given.
- *  - calls are thread-safe: the library holds no mutable global state besides
- *    the thread-local error string.
+ *  - calls are thread-safe with one exception: the tuning tables set by
+ *    ptx_tune_set / ptx_tune_set_groups are process-wide.  Each call reads
+ *    them once, into a per-call snapshot used for both its workspace layout
+ *    and its launches; but a workspace size queried before a ptx_tune_set*
+ *    can be too small for a call made after it (the call then fails with
+ *    PTX_ERR_WORKSPACE).  Tune before the work starts, not beside it.  Other
+ *    state: the thread-local error string and timing hook.
  */
 #ifndef PTX_H_
 #define PTX_H_
@@ -180,7 +185,9 @@ ptx_status ptx_base_step(const void* sample, int64_t sample_h, int64_t sample_w,
  * batch is entry order[b] of the full tables positions_all [count][2] and
  * measured_all [count, h, w] (python/train_ptychography.py:532-553 draws
  * torch.randperm each epoch and indexes both tables).  An order entry outside
- * [0, count) reads entry 0 instead.  Workspace: ptx_base_step_workspace. */
+ * [0, count) reads entry 0 instead.  count must lie in [1, 2^31 - 1] when
+ * batch > 0 (PTX_ERR_INVALID_ARGUMENT otherwise).  Workspace:
+ * ptx_base_step_workspace. */
 ptx_status ptx_base_step_indexed(const void* sample, int64_t sample_h, int64_t sample_w,
                                  const void* probe, int64_t probe_h, int64_t probe_w,
                                  const int64_t* positions_all, const float* measured_all, int64_t count,
@@ -207,7 +214,8 @@ ptx_status ptx_adam_step(float* param, const float* grad, float* exp_avg, float*
 
 /* positions_out[b] = positions_all[order[b]] (int64 pairs); entries of order
  * outside [0, count) read entry 0 and set *bad_index_flag to 1 (the caller
- * zeroes the flag). */
+ * zeroes the flag).  An empty table (count 0) with batch > 0 is
+ * PTX_ERR_INVALID_ARGUMENT. */
 ptx_status ptx_gather_positions(const int64_t* order, const int64_t* positions_all, int64_t count, int64_t batch,
                                 int64_t* positions_out, int* bad_index_flag, void* stream);
 

@@ -132,7 +132,10 @@ __device__ __forceinline__ int64_t pmod(int64_t a, int64_t n) {
 
 // For every grid tile, the views whose window overlaps it, in ascending view
 // order.  lists: [ntiles, batch] int32; counts: [ntiles].  One 256-thread
-// work-group per tile.
+// work-group per tile.  With inside_only (scan windows, which must not wrap)
+// a view whose window leaves the grid is listed nowhere: the scatter writes
+// no contribution for it, so its rows of the contribution stack hold stale
+// data that the gather must never read.
 struct TileLists {
   const int64_t* pos;
   int64_t batch;
@@ -142,6 +145,7 @@ struct TileLists {
   int* lists;
   int* counts;
   int64_t ntiles;  // 0: nothing to do
+  int inside_only;
   __host__ __device__ int64_t blocks() const { return ntiles; }
   __device__ __forceinline__ void run(int64_t tile, int* wave_tot /* 4 ints of LDS */) const {
     const int64_t ty0 = (tile / tiles_x) * tile_h, tx0 = (tile % tiles_x) * kTileW;
@@ -151,11 +155,12 @@ struct TileLists {
       const int64_t b = base + threadIdx.x;
       bool hit = false;
       if (b < batch) {
-        const int64_t oy = pmod(pos[2 * b + 1] + off_y, H), ox = pmod(pos[2 * b] + off_x, W);
+        const int64_t px = pos[2 * b], py = pos[2 * b + 1];
+        const int64_t oy = pmod(py + off_y, H), ox = pmod(px + off_x, W);
         // two arcs on a circle overlap iff one starts inside the other
         const bool ry = pmod(ty0 - oy, H) < h || pmod(oy - ty0, H) < tile_h;
         const bool rx = pmod(tx0 - ox, W) < w || pmod(ox - tx0, W) < kTileW;
-        hit = ry && rx;
+        hit = ry && rx && (!inside_only || window_ok(px, py, h, w, H, W));
       }
       const unsigned long long m = __ballot(hit);
       const int before = __popcll(m & ((1ull << lane) - 1ull));
@@ -476,26 +481,26 @@ inline GatherScratch carve_gather(Carver& cv, int64_t H, int64_t W, int64_t h, i
 // pos + (off_x, off_y), wrapping modulo (H, W)): tile lists, segment plan,
 // segmented gather, combine.
 // The list job of a gather (to run as side work of an earlier launch).
+// inside_only: the windows are scan windows (no wrap-around); a view whose
+// window leaves the grid contributes nothing.
 inline TileLists gather_lists_work(const GatherScratch& g, const int64_t* pos, int64_t B, int64_t h, int64_t w,
-                                   int64_t H, int64_t W, int64_t off_y, int64_t off_x) {
+                                   int64_t H, int64_t W, int64_t off_y, int64_t off_x, bool inside_only) {
   const TileGrid tg = tile_grid(h, w, B, H, W);
-  return TileLists{pos, B, (int)h, (int)w, H, W, off_y, off_x, tg.tiles_x, tg.tile_h, g.lists, g.counts, tg.count()};
+  return TileLists{pos,           B,        (int)h,   (int)w,    H,          W,
+                   off_y,         off_x,    tg.tiles_x, tg.tile_h, g.lists,    g.counts,
+                   tg.count(),    inside_only ? 1 : 0};
 }
 
-// grad (+)= the windows' contributions: tile lists (unless lists_done: an
-// earlier launch ran gather_lists_work), then the single-pass gather or plan,
-// segmented gather and combine.  `side` rides along with the gather launch.
+// grad (+)= the windows' contributions, from the tile lists an earlier
+// launch built (gather_lists_work as its side work): the single-pass gather,
+// or plan, segmented gather and combine.  `side` rides along with the gather
+// launch.
 inline ptx_status run_tile_gather(const GatherScratch& g, const float2* contrib, int64_t h, int64_t w,
                                   const int64_t* pos, int64_t B, int64_t off_y, int64_t off_x, float2* grad,
-                                  int64_t H, int64_t W, int accumulate, hipStream_t s, bool lists_done = false,
+                                  int64_t H, int64_t W, int accumulate, hipStream_t s,
                                   const SideWork& side = SideWork{}) {
   const TileGrid tg = tile_grid(h, w, B, H, W);
   ptx_status st;
-  if (!lists_done) {
-    const TileLists tl = gather_lists_work(g, pos, B, h, w, H, W, off_y, off_x);
-    k_tile_lists<<<dim3((unsigned)tl.blocks()), dim3(256), 0, s>>>(tl);
-    if ((st = check_launch("tile lists", s)) != PTX_OK) return st;
-  }
   if (!gather_segmented(h, w, B, H, W)) {
     k_tile_gather<false><<<dim3((unsigned)(tg.count() + side.blocks())), dim3(256), 0, s>>>(
         contrib, (int)h, (int)w, pos, B, off_y, off_x, g.lists, g.counts, tg.tiles_x, nullptr, nullptr, nullptr,

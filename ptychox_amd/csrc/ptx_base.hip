@@ -289,6 +289,7 @@ extern "C" {
 
 ptx_status ptx_base_forward(const void* sample, int64_t H, int64_t W, const void* probe, int64_t h, int64_t w,
                             const int64_t* pos, int64_t B, void* far_field, float* amplitude, void* stream) {
+  TuneScope tune_scope;
   ptx_status st = check_shapes("ptx_base_forward", H, W, h, w, B);
   if (st != PTX_OK) return st;
   if (B == 0) return PTX_OK;
@@ -306,6 +307,7 @@ ptx_status ptx_base_forward(const void* sample, int64_t H, int64_t W, const void
 }
 
 size_t ptx_base_backward_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
+  TuneScope tune_scope;
   if (check_shapes("ptx_base_backward_workspace", H, W, h, w, B) != PTX_OK) return 0;
   ScatterPlan sp{};
   dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
@@ -315,6 +317,7 @@ size_t ptx_base_backward_workspace(int64_t H, int64_t W, int64_t h, int64_t w, i
 ptx_status ptx_base_backward(const float* grad_output, const void* far_field, const void* sample, int64_t H, int64_t W,
                              const void* probe, int64_t h, int64_t w, const int64_t* pos, int64_t B, void* grad_sample,
                              void* grad_probe, void* workspace, size_t ws_bytes, unsigned flags, void* stream) {
+  TuneScope tune_scope;
   ptx_status st = check_shapes("ptx_base_backward", H, W, h, w, B);
   if (st != PTX_OK) return st;
   hipStream_t s = (hipStream_t)stream;
@@ -349,6 +352,7 @@ ptx_status ptx_base_backward(const float* grad_output, const void* far_field, co
 }
 
 size_t ptx_base_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
+  TuneScope tune_scope;
   if (check_shapes("ptx_base_step_workspace", H, W, h, w, B) != PTX_OK) return 0;
   ScatterPlan sp{};
   dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
@@ -423,6 +427,7 @@ ptx_status ptx_base_step(const void* sample, int64_t H, int64_t W, const void* p
                          const int64_t* pos, int64_t B, const float* measured, float loss_scale,
                          float* loss_per_position, float* amplitude, void* grad_sample, void* grad_probe,
                          void* workspace, size_t ws_bytes, unsigned flags, void* stream) {
+  TuneScope tune_scope;
   return base_step_impl("ptx_base_step", sample, H, W, probe, h, w, pos, B, measured, nullptr, B, loss_scale,
                         loss_per_position, amplitude, grad_sample, grad_probe, workspace, ws_bytes, flags,
                         (hipStream_t)stream);
@@ -433,8 +438,15 @@ ptx_status ptx_base_step_indexed(const void* sample, int64_t H, int64_t W, const
                                  const int64_t* order, int64_t B, float loss_scale, float* loss_per_position,
                                  float* amplitude, void* grad_sample, void* grad_probe, void* workspace,
                                  size_t ws_bytes, unsigned flags, void* stream) {
+  TuneScope tune_scope;
   if (B > 0 && !order) {
     set_error("ptx_base_step_indexed: null view order");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  // the kernels index the tables with 32-bit unsigned compares; an empty
+  // table has no entry 0 to fall back on
+  if (B > 0 && (count <= 0 || count > INT32_MAX)) {
+    set_error("ptx_base_step_indexed: table of %lld views (need 1 .. 2^31 - 1)", (long long)count);
     return PTX_ERR_INVALID_ARGUMENT;
   }
   return base_step_impl("ptx_base_step_indexed", sample, H, W, probe, h, w, positions_all, count, measured_all,

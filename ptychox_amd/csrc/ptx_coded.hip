@@ -791,6 +791,7 @@ ptx_status ptx_coded_forward(const void* object_field, int64_t H, int64_t W, con
                              const void* transfer_1, const void* transfer_2, const float* shifts_xy, int64_t B,
                              int64_t downsample, void* object_spectrum, void* x_forward, void* sensor_field,
                              float* amplitude, void* workspace, size_t ws_bytes, void* stream) {
+  TuneScope tune_scope;
   ptx_status st = check("ptx_coded_forward", H, W, B, downsample);
   if (st != PTX_OK) return st;
   if (!object_field || !object_spectrum || (B > 0 && (!coded_surface || !transfer_2 || !shifts_xy ||
@@ -825,6 +826,7 @@ ptx_status ptx_coded_forward(const void* object_field, int64_t H, int64_t W, con
 }
 
 size_t ptx_coded_workspace(int64_t H, int64_t W, int64_t B, int64_t downsample, int step) {
+  TuneScope tune_scope;
   if (check("ptx_coded_workspace", H, W, B, downsample) != PTX_OK) return 0;
   return carve(nullptr, H, W, B, downsample, step != 0, plan(H, W, B)).bytes;
 }
@@ -834,6 +836,7 @@ ptx_status ptx_coded_backward(const float* grad_output, const void* object_spect
                               const void* transfer_2, const float* shifts_xy, int64_t B, int64_t H, int64_t W,
                               int64_t downsample, void* grad_object, void* grad_coded, void* workspace,
                               size_t ws_bytes, unsigned flags, void* stream) {
+  TuneScope tune_scope;
   ptx_status st = check("ptx_coded_backward", H, W, B, downsample);
   if (st != PTX_OK) return st;
   if (!grad_object || !grad_coded) {
@@ -886,6 +889,7 @@ ptx_status ptx_coded_step(const void* object_field, int64_t H, int64_t W, const 
                           int64_t downsample, const float* measured, float loss_scale, float* loss_per_position,
                           float* amplitude, void* grad_object, void* grad_coded, void* workspace, size_t ws_bytes,
                           unsigned flags, void* stream) {
+  TuneScope tune_scope;
   ptx_status st = check("ptx_coded_step", H, W, B, downsample);
   if (st != PTX_OK) return st;
   if (!grad_object || !grad_coded) {

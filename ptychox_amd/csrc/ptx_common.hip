@@ -28,17 +28,34 @@ static int default_elems(int kind, int64_t n) {
 
 static std::atomic<int> g_groups[PASS_KINDS][16];  // 0 = built-in default
 
+// per-thread snapshot of both tables, live while a TuneScope is open
+static thread_local int t_snap_elems[PASS_KINDS][16];
+static thread_local int t_snap_groups[PASS_KINDS][16];
+static thread_local int t_scope_depth = 0;
+
+TuneScope::TuneScope() {
+  if (t_scope_depth++ > 0) return;
+  for (int k = 0; k < PASS_KINDS; ++k)
+    for (int l = 0; l < 16; ++l) {
+      t_snap_elems[k][l] = g_tune[k][l].load(std::memory_order_relaxed);
+      t_snap_groups[k][l] = g_groups[k][l].load(std::memory_order_relaxed);
+    }
+}
+TuneScope::~TuneScope() { --t_scope_depth; }
+
 int tuned_groups(int kind, int64_t n, int def) {
   if (kind < 0 || kind >= PASS_KINDS || !is_pow2(n)) return def;
   const int l = ilog2((int)n);
-  const int v = (l >= 0 && l < 16) ? g_groups[kind][l].load(std::memory_order_relaxed) : 0;
+  if (l < 0 || l >= 16) return def;
+  const int v = t_scope_depth > 0 ? t_snap_groups[kind][l] : g_groups[kind][l].load(std::memory_order_relaxed);
   return v ? v : def;
 }
 
 int tuned_elems(int kind, int64_t n) {
   if (kind < 0 || kind >= PASS_KINDS || !is_pow2(n)) return default_elems(kind, n);
   const int l = ilog2((int)n);
-  const int v = (l >= 0 && l < 16) ? g_tune[kind][l].load(std::memory_order_relaxed) : 0;
+  if (l < 0 || l >= 16) return default_elems(kind, n);
+  const int v = t_scope_depth > 0 ? t_snap_elems[kind][l] : g_tune[kind][l].load(std::memory_order_relaxed);
   return v ? v : default_elems(kind, n);
 }
 
@@ -175,6 +192,7 @@ const char* ptx_version(void) { return "ptychox_hip 0.1.0 (gfx950)"; }
 int ptx_abi_version(void) { return PTX_ABI_VERSION; }
 
 ptx_status ptx_fft2d(void* data, int64_t batch, int64_t h, int64_t w, int inverse, void* stream) {
+  TuneScope tune_scope;
   g_last_error[0] = 0;
   if (batch < 0 || (batch > 0 && data == nullptr)) {
     set_error("ptx_fft2d: invalid data pointer or batch");

@@ -511,7 +511,7 @@ ptx_status fpm_tail(const FpmWorkspace& ws, const Plan& p, const float2* spec, i
   }
   const float inv_n = 1.0f / (float)(h * w);
   SideWork with_scatter{};
-  if (det) with_scatter.lists = gather_lists_work(ws.gather, led, B, h, w, H, W, H / 2, W / 2);
+  if (det) with_scatter.lists = gather_lists_work(ws.gather, led, B, h, w, H, W, H / 2, W / 2, false);
   with_scatter.loss = loss;
   st = dispatch_ne<ScatterRun>(PASS_ROW_LOOP, w, ws.tb, spec, H, W, pupil, (int)h, led, B, p, inv_n, ws.gspec,
                               det ? ws.tb : nullptr, ws.gq_part, with_scatter, s);
@@ -520,7 +520,7 @@ ptx_status fpm_tail(const FpmWorkspace& ws, const Plan& p, const float2* spec, i
   if (det) {
     SideWork with_gather{};
     with_gather.sums = pupil_sum;
-    st = run_tile_gather(ws.gather, ws.tb, h, w, led, B, H / 2, W / 2, ws.gspec, H, W, 0, s, true, with_gather);
+    st = run_tile_gather(ws.gather, ws.tb, h, w, led, B, H / 2, W / 2, ws.gspec, H, W, 0, s, with_gather);
     if (st != PTX_OK) return st;
   } else {
     sum_parts2(ws.gq_part, p.nchunks, h * w, gpupil, acc ? 1 : 0, s);
@@ -542,6 +542,7 @@ extern "C" {
 ptx_status ptx_fpm_forward(const void* object_field, int64_t H, int64_t W, const void* pupil, int64_t h, int64_t w,
                            const int64_t* led, int64_t B, void* object_fft, void* image_field, float* amplitude,
                            void* stream) {
+  TuneScope tune_scope;
   ptx_status st = check("ptx_fpm_forward", H, W, h, w, B);
   if (st != PTX_OK) return st;
   if (!object_field || !pupil || !object_fft || (B > 0 && (!led || !image_field || !amplitude))) {
@@ -560,6 +561,7 @@ ptx_status ptx_fpm_forward(const void* object_field, int64_t H, int64_t W, const
 }
 
 size_t ptx_fpm_backward_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
+  TuneScope tune_scope;
   if (check("ptx_fpm_backward_workspace", H, W, h, w, B) != PTX_OK) return 0;
   Plan p{};
   dispatch_ne<PlanQuery>(PASS_ROW_LOOP, w, h, B, &p);
@@ -570,6 +572,7 @@ ptx_status ptx_fpm_backward(const float* grad_output, const void* image_field, c
                             int64_t W, const void* pupil, int64_t h, int64_t w, const int64_t* led, int64_t B,
                             void* grad_object, void* grad_pupil, void* workspace, size_t ws_bytes, unsigned flags,
                             void* stream) {
+  TuneScope tune_scope;
   ptx_status st = check("ptx_fpm_backward", H, W, h, w, B);
   if (st != PTX_OK) return st;
   hipStream_t s = (hipStream_t)stream;
@@ -604,6 +607,7 @@ ptx_status ptx_fpm_backward(const float* grad_output, const void* image_field, c
 }
 
 size_t ptx_fpm_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
+  TuneScope tune_scope;
   if (check("ptx_fpm_step_workspace", H, W, h, w, B) != PTX_OK) return 0;
   Plan p{};
   dispatch_ne<PlanQuery>(PASS_ROW_LOOP, w, h, B, &p);
@@ -616,6 +620,7 @@ ptx_status ptx_fpm_step(const void* object_field, int64_t H, int64_t W, const vo
                         const int64_t* led, int64_t B, const float* measured, float loss_scale,
                         float* loss_per_position, float* amplitude, void* grad_object, void* grad_pupil,
                         void* workspace, size_t ws_bytes, unsigned flags, void* stream) {
+  TuneScope tune_scope;
   ptx_status st = check("ptx_fpm_step", H, W, h, w, B);
   if (st != PTX_OK) return st;
   hipStream_t s = (hipStream_t)stream;

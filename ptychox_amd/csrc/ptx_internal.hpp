@@ -67,6 +67,18 @@ int tuned_elems(int kind, int64_t n);
 // work-group walks.  `def` when the table holds no setting for (kind, n).
 int tuned_groups(int kind, int64_t n, int def);
 
+// Consistent view of the tuning tables for the duration of one entry-point
+// call: the first scope on a thread copies both process-wide tables into a
+// thread-local snapshot that tuned_elems / tuned_groups read until it ends,
+// so a ptx_tune_set* on another thread cannot change the schedule between a
+// call's workspace layout and its launches.
+struct TuneScope {
+  TuneScope();
+  ~TuneScope();
+  TuneScope(const TuneScope&) = delete;
+  TuneScope& operator=(const TuneScope&) = delete;
+};
+
 // Call F<N, E>::run(args...) for the runtime power-of-two n and the tuned E.
 template <template <int, int> class F, typename... Args>
 ptx_status dispatch_ne(int kind, int64_t n, Args&&... args) {

@@ -179,6 +179,7 @@ using namespace ptx;
 extern "C" {
 
 size_t ptx_loss_workspace(int64_t batch, int64_t h, int64_t w) {
+  TuneScope tune_scope;
   if (batch < 0 || h < 1 || w < 1) return 0;
   return (size_t)(batch * ((h * w + kLossBlock - 1) / kLossBlock)) * sizeof(float);
 }
@@ -186,6 +187,7 @@ size_t ptx_loss_workspace(int64_t batch, int64_t h, int64_t w) {
 ptx_status ptx_amplitude_loss(int kind, const float* pred, const float* target, int64_t batch, int64_t h, int64_t w,
                               float scale, float eps, float* loss_per_frame, float* grad, void* workspace,
                               size_t workspace_bytes, void* stream) {
+  TuneScope tune_scope;
   return stencil_loss<float>("ptx_amplitude_loss", kind, pred, target, batch, h, w, scale, eps, loss_per_frame, grad,
                              workspace, workspace_bytes, (hipStream_t)stream);
 }
@@ -193,6 +195,7 @@ ptx_status ptx_amplitude_loss(int kind, const float* pred, const float* target, 
 ptx_status ptx_field_regularizer(int kind, const void* field, int64_t batch, int64_t h, int64_t w, float scale,
                                  float eps, float* loss_per_frame, void* grad, void* workspace,
                                  size_t workspace_bytes, void* stream) {
+  TuneScope tune_scope;
   return stencil_loss<float2>("ptx_field_regularizer", kind, (const float2*)field, nullptr, batch, h, w, scale, eps,
                               loss_per_frame, (float2*)grad, workspace, workspace_bytes, (hipStream_t)stream);
 }

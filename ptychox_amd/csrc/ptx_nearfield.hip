@@ -170,6 +170,7 @@ extern "C" {
 ptx_status ptx_nearfield_forward(const void* sample, int64_t H, int64_t W, const void* probe, int64_t h, int64_t w,
                                  const void* transfer, const int64_t* pos, int64_t B, void* field, float* amplitude,
                                  void* stream) {
+  TuneScope tune_scope;
   ptx_status st = nf_check("ptx_nearfield_forward", H, W, h, w, B, transfer);
   if (st != PTX_OK || B == 0) return st;
   if (!sample || !probe || !pos || !field || !amplitude) {
@@ -187,6 +188,7 @@ ptx_status ptx_nearfield_forward(const void* sample, int64_t H, int64_t W, const
 }
 
 size_t ptx_nearfield_backward_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
+  TuneScope tune_scope;
   if (check_shapes("ptx_nearfield_backward_workspace", H, W, h, w, B) != PTX_OK) return 0;
   ScatterPlan sp{};
   dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
@@ -197,6 +199,7 @@ ptx_status ptx_nearfield_backward(const float* grad_output, const void* field, c
                                   int64_t W, const void* probe, int64_t h, int64_t w, const void* transfer,
                                   const int64_t* pos, int64_t B, void* grad_sample, void* grad_probe, void* workspace,
                                   size_t ws_bytes, unsigned flags, void* stream) {
+  TuneScope tune_scope;
   ptx_status st = nf_check("ptx_nearfield_backward", H, W, h, w, B, transfer);
   if (st != PTX_OK) return st;
   hipStream_t s = (hipStream_t)stream;
@@ -231,6 +234,7 @@ ptx_status ptx_nearfield_backward(const float* grad_output, const void* field, c
 }
 
 size_t ptx_nearfield_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64_t B, unsigned flags) {
+  TuneScope tune_scope;
   if (check_shapes("ptx_nearfield_step_workspace", H, W, h, w, B) != PTX_OK) return 0;
   ScatterPlan sp{};
   dispatch_ne<ScatterPlanQuery>(PASS_ROW_LOOP, w, h, B, &sp);
@@ -241,6 +245,7 @@ ptx_status ptx_nearfield_step(const void* sample, int64_t H, int64_t W, const vo
                               const void* transfer, const int64_t* pos, int64_t B, const float* measured,
                               float loss_scale, float* loss_per_position, float* amplitude, void* grad_sample,
                               void* grad_probe, void* workspace, size_t ws_bytes, unsigned flags, void* stream) {
+  TuneScope tune_scope;
   ptx_status st = nf_check("ptx_nearfield_step", H, W, h, w, B, transfer);
   if (st != PTX_OK) return st;
   hipStream_t s = (hipStream_t)stream;

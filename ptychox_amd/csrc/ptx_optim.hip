@@ -108,6 +108,10 @@ ptx_status ptx_gather_positions(const int64_t* order, const int64_t* positions_a
     set_error("ptx_gather_positions: invalid arguments");
     return PTX_ERR_INVALID_ARGUMENT;
   }
+  if (batch > 0 && count == 0) {  // no entry 0 to fall back on
+    set_error("ptx_gather_positions: empty position table for a batch of %lld", (long long)batch);
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
   return launch_gather_positions(order, positions_all, count, batch, positions_out, bad_index_flag,
                                  (hipStream_t)stream);
 }

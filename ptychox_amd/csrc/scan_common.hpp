@@ -317,7 +317,7 @@ ptx_status scan_scatter(const ScanWorkspace& ws, const ScatterPlan& sp, const fl
     if ((st = check_launch("zero fill", s)) != PTX_OK) return st;
   }
   SideWork with_scatter{};
-  if (det) with_scatter.lists = gather_lists_work(ws.gather, pos, B, h, w, H, W, 0, 0);
+  if (det) with_scatter.lists = gather_lists_work(ws.gather, pos, B, h, w, H, W, 0, 0, true);
   with_scatter.loss = loss;
   st = dispatch_ne<ScatterRowsRun>(PASS_ROW_LOOP, w, ws.tb, sample, H, W, probe, (int)h, pos, B, sp, gs,
                                   det ? ws.tb : nullptr, ws.gp_part, scale, with_scatter, s,
@@ -330,7 +330,7 @@ ptx_status scan_scatter(const ScanWorkspace& ws, const ScatterPlan& sp, const fl
   }
   SideWork with_gather{};
   with_gather.sums = probe_sum;
-  return run_tile_gather(ws.gather, ws.tb, h, w, pos, B, 0, 0, gs, H, W, acc ? 1 : 0, s, true, with_gather);
+  return run_tile_gather(ws.gather, ws.tb, h, w, pos, B, 0, 0, gs, H, W, acc ? 1 : 0, s, with_gather);
 }
 
 }  // namespace

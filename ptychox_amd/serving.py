@@ -229,6 +229,24 @@ def footprint(req: Request) -> int:
 
 # ----------------------------------------------------------------------------- device-resident job
 
+def hyper_table(lr: float, epochs: int) -> torch.Tensor:
+    """Adam step sizes of a request, one row per epoch: row e holds
+    ``adam_hyper`` of epoch e + 1 (cosine learning rate, one step per epoch),
+    then a pad column.  A last row ``(0, 1, 1, 0)`` -- step size 0, with a
+    bias correction and eps that keep the denominator >= 1 -- leaves the
+    parameters unchanged if an epoch ever runs past the plan."""
+    rows = [list(adam_hyper(cosine_lr(lr, 0.0, epochs, e), 0.9, 0.999, 1e-16, e)) + [0.0]
+            for e in range(1, epochs + 1)]
+    return torch.tensor(rows + [[0.0, 1.0, 1.0, 0.0]], dtype=torch.float32)
+
+
+def epoch_hyper(table: torch.Tensor, counter: torch.Tensor, epochs: int) -> torch.Tensor:
+    """The row of ``table`` for the epoch ``counter`` (a one-element int64
+    device tensor) counts, selected on the device; the index is clamped to
+    the last row, so a counter past the plan never reads beyond the table."""
+    return table.index_select(0, counter.clamp_max(epochs))[0]
+
+
 class Job:
     """One admitted request: its device state, stream and captured epoch."""
 
@@ -266,10 +284,7 @@ class Job:
                 self.step = NearFieldStep(self.obj.shape, self.aux.shape, V, device, scatter="segments")
             else:
                 self.step = CodedStep(self.obj.shape, V, 1, device)
-            # row e: Adam step sizes of epoch e + 1 (cosine learning rate, one step per epoch)
-            rows = [list(adam_hyper(cosine_lr(req.lr, 0.0, req.epochs, e), 0.9, 0.999, 1e-16, e)) + [0.0]
-                    for e in range(1, req.epochs + 1)]
-            self.hyper_table = torch.tensor(rows + [[0.0] * 4], dtype=torch.float32).to(device)
+            self.hyper_table = hyper_table(req.lr, req.epochs).to(device)
             self.counter = torch.zeros(1, dtype=torch.int64, device=device)
         self.resident_bytes = torch.cuda.memory_allocated(device) - before
 
@@ -284,11 +299,11 @@ class Job:
                       self.g_aux)
 
     def _launch_epoch(self) -> None:
-        # the step-size table has one row per epoch: never index past it
-        if self.launched >= self.req.epochs:
-            raise RuntimeError(f"request {self.req.rid} has already run its {self.req.epochs} epochs")
-        # the step sizes of the current epoch, selected on the device
-        hyper = self.hyper_table.index_select(0, self.counter)[0]
+        # The step sizes of the current epoch, selected on the device.  The
+        # table has epochs + 1 rows (the last one all zeros); the index is
+        # clamped on the device too, so even a replay past the planned epochs
+        # reads the zero row instead of memory past the table.
+        hyper = epoch_hyper(self.hyper_table, self.counter, self.req.epochs)
         self._fused_step()
         self.opt_o.step(self.g_obj, hyper)
         self.opt_a.step(self.g_aux, hyper)
@@ -305,26 +320,35 @@ class Job:
             g.capture_end()
         self.graph = g
 
+    def launch_next(self) -> None:
+        """Queue the request's next epoch on its stream: eagerly for the
+        first, then as a captured graph (captured at the second epoch and
+        replayed after).  Raises RuntimeError, before any launch, once the
+        request has run all its epochs."""
+        if self.launched >= self.req.epochs:
+            raise RuntimeError(f"request {self.req.rid} has already run its {self.req.epochs} epochs")
+        with torch.cuda.stream(self.stream):
+            if not self.graph_enabled or self.launched == 0:
+                self._launch_epoch()
+            else:
+                if self.graph is None:
+                    self._capture()
+                self.graph.replay()
+            ev = torch.cuda.Event()
+            ev.record(self.stream)
+        self.inflight.append(ev)
+        self.launched += 1
+        if self.launched == self.req.epochs:
+            self.done_event = ev
+
     def pump(self) -> bool:
         """Queue epochs until ``depth`` are in flight; True if any was queued."""
         while self.inflight and self.inflight[0].query():
             self.inflight.popleft()
         queued = False
         while self.launched < self.req.epochs and len(self.inflight) < self.depth:
-            with torch.cuda.stream(self.stream):
-                if not self.graph_enabled or self.launched == 0:
-                    self._launch_epoch()
-                else:
-                    if self.graph is None:
-                        self._capture()
-                    self.graph.replay()
-                ev = torch.cuda.Event()
-                ev.record(self.stream)
-            self.inflight.append(ev)
-            self.launched += 1
+            self.launch_next()
             queued = True
-            if self.launched == self.req.epochs:
-                self.done_event = ev
         return queued
 
     @property

@@ -97,6 +97,21 @@ def test_adam_argument_checks(lib):
     lib.call("ptx_adam_step", *args, 0, 0.9, 0.999, fake, 0, None)  # n = 0: nothing to do
 
 
+def test_view_table_checks(lib):
+    # an empty (or 2^31-entry) table for a non-empty batch is rejected on the
+    # host, before anything is launched
+    fake = ctypes.c_void_p(256)
+    for count in (0, -1, 1 << 31):
+        with pytest.raises(ValueError, match="table of"):
+            lib.call("ptx_base_step_indexed", fake, 256, 256, fake, 64, 64, fake, fake, count, fake, 4, 1.0,
+                     fake, None, fake, fake, fake, 1 << 30, 0, None)
+    with pytest.raises(ValueError, match="empty position table"):
+        lib.call("ptx_gather_positions", fake, fake, 0, 4, fake, fake, None)
+    with pytest.raises(ValueError, match="invalid arguments"):
+        lib.call("ptx_gather_positions", fake, fake, -1, 4, fake, fake, None)
+    lib.call("ptx_gather_positions", fake, fake, 0, 0, fake, fake, None)  # empty batch: nothing to do
+
+
 def test_timing_hook_arms_and_disarms(lib):
     lib.call("ptx_timing_events", None, 0)
     assert lib.lib().ptx_timing_count() == 0

@@ -75,3 +75,26 @@ def test_tracks_plain_torch_loop(gpu, problems, idx):
     rel = abs(fused.final_loss - plain.final_loss) / abs(plain.final_loss)
     print(req.kind, fused.final_loss, plain.final_loss, rel, flush=True)
     assert rel < 2e-3
+
+
+def test_finished_job_refuses_another_epoch(gpu, problems):
+    # the host guard: a finished request raises before anything is launched
+    req = MIX[6]
+    job = sv.Job(req, problems[req.rid], gpu, torch.cuda.Stream(device=gpu), graph=True, depth=req.epochs + 1)
+    while job.launched < req.epochs:
+        job.launch_next()
+    torch.cuda.synchronize()
+    obj, aux = job.obj.clone(), job.aux.clone()
+    assert int(job.counter) == req.epochs
+    with pytest.raises(RuntimeError, match="already run"):
+        job.launch_next()
+    torch.cuda.synchronize()
+    assert int(job.counter) == req.epochs and job.launched == req.epochs
+    assert torch.equal(job.obj, obj) and torch.equal(job.aux, aux)
+    # the device clamp: a replay past the plan (bypassing the guard) reads the
+    # table's last row, whose step size 0 leaves the unknowns unchanged
+    with torch.cuda.stream(job.stream):
+        job.graph.replay()
+    torch.cuda.synchronize()
+    assert int(job.counter) == req.epochs + 1
+    assert torch.equal(job.obj, obj) and torch.equal(job.aux, aux)

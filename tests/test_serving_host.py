@@ -76,3 +76,27 @@ def test_problems_match_declared_shapes(kind, n, views):
     # deterministic from the request's seed
     q = sv.make_problem(r)
     assert torch.equal(p.arrays["measured"], q.arrays["measured"])
+
+
+def test_step_size_table_index_is_clamped():
+    # The table has one row per epoch plus a harmless last row; a counter past
+    # the plan (a stray graph replay) reads that row, never beyond the table.
+    from ptychox_amd.train import adam_hyper, cosine_lr
+
+    epochs = 5
+    table = sv.hyper_table(1e-2, epochs)
+    assert table.shape == (epochs + 1, 4)
+    for e in range(epochs):
+        row = sv.epoch_hyper(table, torch.tensor([e]), epochs)
+        want = adam_hyper(cosine_lr(1e-2, 0.0, epochs, e + 1), 0.9, 0.999, 1e-16, e + 1)
+        assert torch.allclose(row[:3], torch.tensor(want, dtype=torch.float32))
+    for c in (epochs, epochs + 1, epochs + 7, 10**6):
+        assert torch.equal(sv.epoch_hyper(table, torch.tensor([c]), epochs), table[epochs])
+    # Adam's arithmetic with the last row: step size 0 and a denominator >= 1,
+    # so the parameter is unchanged whatever the moments hold (v = 0 included)
+    step, bc2, eps = (float(x) for x in table[epochs, :3])
+    p = torch.tensor([1.5, -2.0, 0.0])
+    m = torch.tensor([0.3, 0.0, -1e30])
+    v = torch.tensor([0.0, 4.0, 1e30])
+    upd = p + (-step) * (m / (v.sqrt() / bc2 + eps))
+    assert torch.equal(upd, p)
