@@ -196,6 +196,27 @@ ptx_status ptx_base_step_indexed(const void* sample, int64_t sample_h, int64_t s
                                  void* grad_sample, void* grad_probe,
                                  void* workspace, size_t workspace_bytes, unsigned flags, void* stream);
 
+/* Allocation-free variant of ptx_base_step.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory at all
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc), which therefore cannot provide a workspace either.  Same
+ * arguments and results as ptx_base_step without the workspace: the scratch
+ * space is a static device array of PTX_NOALLOC_SCRATCH_BYTES (192 MiB) that
+ * each GPU holds from the first call on it until the process ends.  A shape
+ * whose ptx_base_step_workspace exceeds that returns PTX_ERR_WORKSPACE before
+ * anything is launched (object 2048^2 / probe 512^2 / batch 64 needs 162 MiB
+ * with the built-in schedule).  The call allocates nothing.  Every call on a
+ * GPU uses that GPU's one array: calls that overlap on one GPU (other streams,
+ * other threads) race and must be serialised by the caller; calls in one
+ * stream are ordered.  Nothing in the array needs to start at zero. */
+#define PTX_NOALLOC_SCRATCH_BYTES ((size_t)192 << 20)
+ptx_status ptx_base_step_noalloc(const void* sample, int64_t sample_h, int64_t sample_w,
+                                 const void* probe, int64_t probe_h, int64_t probe_w,
+                                 const int64_t* positions_xy, int64_t batch,
+                                 const float* measured, float loss_scale,
+                                 float* loss_per_position, float* amplitude,
+                                 void* grad_sample, void* grad_probe, unsigned flags, void* stream);
+
 /* ------------------------------------------------------------------ optimiser
  * One Adam update with torch.optim.Adam's arithmetic (foreach variant) as the
  * reference drivers configure it (python/train_ptychography.py:500-507,

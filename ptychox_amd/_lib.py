@@ -73,6 +73,10 @@ SIGNATURES: dict[str, tuple] = {
         _st,
         [_p, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _p, _i64, _f, _p, _p, _p, _p, _p, _sz, _u, _p],
     ),
+    "ptx_base_step_noalloc": (
+        _st,
+        [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _f, _p, _p, _p, _p, _u, _p],
+    ),
     "ptx_adam_step": (_st, [_p, _p, _p, _p, _i64, _f, _f, _p, _u, _p]),
     "ptx_loss_workspace": (_sz, [_i64, _i64, _i64]),
     "ptx_amplitude_loss": (_st, [_i, _p, _p, _i64, _i64, _i64, _f, _f, _p, _p, _p, _sz, _p]),

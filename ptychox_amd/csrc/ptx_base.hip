@@ -365,11 +365,11 @@ size_t ptx_base_step_workspace(int64_t H, int64_t W, int64_t h, int64_t w, int64
 }  // extern "C"
 
 namespace ptx {
-namespace {
 
-// Shared body of ptx_base_step and ptx_base_step_indexed.  order == null: the
-// batch's own positions / measured frames; otherwise view b is order[b] of the
-// full tables positions_all[count] / measured_all[count].
+// Shared body of ptx_base_step, ptx_base_step_indexed and
+// ptx_base_step_noalloc.  order == null: the batch's own positions / measured
+// frames; otherwise view b is order[b] of the full tables
+// positions_all[count] / measured_all[count].
 ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t W, const void* probe, int64_t h,
                           int64_t w, const int64_t* pos, int64_t count, const float* measured, const int64_t* order,
                           int64_t B, float loss_scale, float* loss_per_position, float* amplitude, void* grad_sample,
@@ -418,7 +418,6 @@ ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t
                             (float2*)grad_sample, (float2*)grad_probe, flags, 1.0f, s, loss);
 }
 
-}  // namespace
 }  // namespace ptx
 
 extern "C" {

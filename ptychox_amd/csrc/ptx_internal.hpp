@@ -79,6 +79,13 @@ struct TuneScope {
   TuneScope& operator=(const TuneScope&) = delete;
 };
 
+// Body of the conventional fused step (ptx_base.hip), shared by its entry
+// points: ws_bytes of caller scratch at workspace (see ptx_base_step).
+ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t W, const void* probe, int64_t h,
+                          int64_t w, const int64_t* pos, int64_t count, const float* measured, const int64_t* order,
+                          int64_t B, float loss_scale, float* loss_per_position, float* amplitude, void* grad_sample,
+                          void* grad_probe, void* workspace, size_t ws_bytes, unsigned flags, hipStream_t s);
+
 // Call F<N, E>::run(args...) for the runtime power-of-two n and the tuned E.
 template <template <int, int> class F, typename... Args>
 ptx_status dispatch_ne(int kind, int64_t n, Args&&... args) {

@@ -112,6 +112,16 @@ def test_view_table_checks(lib):
     lib.call("ptx_gather_positions", fake, fake, 0, 0, fake, fake, None)  # empty batch: nothing to do
 
 
+def test_noalloc_step_rejects_oversize_without_gpu(lib):
+    # the scratch bound is checked on the host before the scratch array is looked up
+    fake = ctypes.c_void_p(256)
+    with pytest.raises(lib.NativeLibraryError, match="static maximum"):
+        lib.call("ptx_base_step_noalloc", fake, 2048, 2048, fake, 512, 512, fake, 128, fake, 1.0, fake, None,
+                 fake, fake, 0, None)
+    assert lib.lib().ptx_base_step_noalloc(None, 64, 64, None, 24, 24, None, 1, None, 1.0, None, None, None, None,
+                                           0, None) == 2  # unsupported probe side
+
+
 def test_timing_hook_arms_and_disarms(lib):
     lib.call("ptx_timing_events", None, 0)
     assert lib.lib().ptx_timing_count() == 0
