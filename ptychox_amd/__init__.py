@@ -6,7 +6,7 @@ near-field operator.  ``use_cuda_kernels=True`` on complex64 GPU tensors runs
 the hand-written HIP kernels of ``libptychox_hip.so``.
 """
 
-from ._utils import matlab_positions_to_torch, scatter_mode, set_scatter_mode
+from ._utils import matlab_positions_to_torch, recompute_mode, scatter_mode, set_recompute, set_scatter_mode
 from .base import clear_cuda_plan_cache as clear_base_cuda_plan_cache
 from .base import ptychoX_base, ptychography_base
 from .coded import clear_cuda_plan_cache as clear_coded_cuda_plan_cache
@@ -24,6 +24,29 @@ def clear_cuda_plan_cache() -> None:
     clear_fpm_cuda_plan_cache()
     clear_coded_cuda_plan_cache()
     clear_nearfield_cuda_plan_cache()
+
+
+def install_as_ptychoX() -> None:
+    """Make this package importable under the reference's name, so code
+    written against it runs unchanged: ``import ptychoX``, ``from ptychoX.ops
+    import *`` (python/ops.py:3) and ``ptychoX.{base,fpm,coded}.ops`` resolve to
+    this package and its modules.  Refuses (ImportError) when a different
+    ``ptychoX`` is already imported."""
+    import sys
+
+    from . import base, coded, fpm, nearfield, ops
+
+    pkg = sys.modules[__name__]
+    cur = sys.modules.get("ptychoX")
+    if cur is not None and cur is not pkg:
+        raise ImportError("another package named ptychoX is already imported")
+    sys.modules["ptychoX"] = pkg
+    sys.modules["ptychoX.ops"] = ops
+    for name, mod in (("base", base), ("fpm", fpm), ("coded", coded), ("nearfield", nearfield)):
+        sys.modules[f"ptychoX.{name}"] = mod
+        sys.modules[f"ptychoX.{name}.ops"] = mod
+        if not hasattr(mod, "ops"):  # `import ptychoX.base.ops as m` reads the attribute
+            mod.ops = mod
 
 
 __all__ = [
@@ -46,4 +69,7 @@ __all__ = [
     "ptychography_base",
     "scatter_mode",
     "set_scatter_mode",
+    "recompute_mode",
+    "set_recompute",
+    "install_as_ptychoX",
 ]

@@ -65,9 +65,16 @@ def check_windows(positions_xy: torch.Tensor, h: int, w: int, H: int, W: int) ->
 
     The reference has no such check (its torch slice shrinks and ``torch.stack``
     fails; its CUDA kernel reads out of bounds, python/ptychoX/csrc/base_kernel.cu:108-137).
-    One 4-number device-to-host copy when the positions live on the GPU.
+    One 4-number device-to-host copy when the positions live on the GPU; none
+    when they live on the CPU, and no check at all while the current stream is
+    being captured into a graph.
     """
     if positions_xy.numel() == 0:
+        return
+    if positions_xy.is_cuda and torch.cuda.is_current_stream_capturing():
+        # no host synchronisation inside a stream capture: the library's own
+        # contract applies (an out-of-range window gives a zero frame and no
+        # gradient, without touching memory outside the sample)
         return
     ext = torch.stack(
         [positions_xy[:, 0].min(), positions_xy[:, 1].min(), positions_xy[:, 0].max(), positions_xy[:, 1].max()]
@@ -113,3 +120,59 @@ def matlab_positions_to_torch(positions_xy_1based: torch.Tensor) -> torch.Tensor
     """Convert MATLAB one-based ``(x, y)`` scan positions to zero-based int64
     (``python/ptychoX/_utils.py:99-101``)."""
     return positions_xy_1based.to(torch.long) - 1
+
+
+# ----------------------------------------------------------------------------- recompute instead of store
+
+_RECOMPUTE: str | int = "auto"
+
+
+def set_recompute(mode: str | int | None) -> None:
+    """What the native autograd operators keep for their backward.
+
+    The reference saves each call's per-pattern FFT outputs (far field, image
+    field, near-field field, coded ``x_forward`` and ``sensor_field``:
+    python/ptychoX/base/ops.py:69, fpm/ops.py:62, coded/ops.py:108).  Here they
+    are either saved or recomputed from the saved inputs in the backward
+    (bitwise the same values: the kernels are deterministic).  ``mode``:
+
+    * ``"never"``: always save them (the reference's behaviour);
+    * ``"always"``: always recompute;
+    * an int ``n``: recompute when they would take more than ``n`` bytes;
+    * ``"auto"`` (default, also for ``None``): recompute when they would take
+      more than half of the device memory free at the forward call.
+    """
+    global _RECOMPUTE
+    if mode is None:
+        mode = "auto"
+    if isinstance(mode, bool) or not (isinstance(mode, int) or mode in ("never", "always", "auto")):
+        raise ValueError(f"recompute mode must be 'never', 'always', 'auto' or a byte budget, got {mode!r}")
+    if isinstance(mode, int) and mode < 0:
+        raise ValueError("recompute byte budget must be >= 0")
+    _RECOMPUTE = mode
+
+
+def recompute_mode() -> str | int:
+    return _RECOMPUTE
+
+
+def should_recompute(saved_bytes: int, device: torch.device, free_bytes: int | None = None) -> bool:
+    """True when a forward that would save ``saved_bytes`` of per-pattern
+    outputs should keep only its inputs (see :func:`set_recompute`)."""
+    m = recompute_mode()
+    if m == "never":
+        return False
+    if m == "always":
+        return True
+    if isinstance(m, int):
+        return saved_bytes > m
+    if free_bytes is None:
+        if device.type != "cuda" or torch.cuda.is_current_stream_capturing():
+            return False  # no device query inside a graph capture: keep the outputs
+        free_bytes, _total = torch.cuda.mem_get_info(device)
+    return saved_bytes > free_bytes // 2
+
+
+_env = os.environ.get("PTYCHOX_RECOMPUTE")
+if _env:
+    set_recompute(int(_env) if _env.isdigit() else _env)

@@ -16,8 +16,9 @@ from ._utils import (
     can_use_native,
     check_complex_scan_inputs,
     check_windows,
-    scatter_flags,
     require_native_frame,
+    scatter_flags,
+    should_recompute,
 )
 
 
@@ -121,19 +122,24 @@ class PtychoXBaseFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, sample, probe, positions_xy, use_cuda_kernels=False):  # type: ignore[override]
         check_complex_scan_inputs(sample, probe, positions_xy)
+        h, w = probe.shape
+        # checked before the move to the device: CPU positions cost no synchronisation
+        check_windows(positions_xy, h, w, *sample.shape)
         positions_xy = positions_xy.to(device=sample.device, dtype=torch.long).contiguous()
         sample = sample.contiguous()
         probe = probe.contiguous()
-        h, w = probe.shape
         ctx.sample_shape = tuple(sample.shape)
+        ctx.recompute = False
         if can_use_native(sample, probe, use_cuda_kernels):
             require_native_frame(h, w, "probe")
-            check_windows(positions_xy, h, w, *sample.shape)
             far_field, amplitude = native_forward(sample, probe, positions_xy)
-            ctx.save_for_backward(sample, probe, far_field, positions_xy)
+            # the far field is the one per-pattern tensor kept for the backward
+            # (the patches are re-gathered from the sample); recomputed instead
+            # when it does not fit (python/ptychoX/base/ops.py:69 saves both)
+            ctx.recompute = should_recompute(far_field.numel() * 8, sample.device)
+            ctx.save_for_backward(sample, probe, None if ctx.recompute else far_field, positions_xy)
             ctx.native = True
         else:
-            check_windows(positions_xy, h, w, *sample.shape)
             patches, far_field, amplitude = torch_forward(sample, probe, positions_xy)
             ctx.save_for_backward(patches, probe, far_field, positions_xy)
             ctx.native = False
@@ -142,8 +148,10 @@ class PtychoXBaseFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_output):  # type: ignore[override]
         first, probe, far_field, positions_xy = ctx.saved_tensors
-        grad_output = grad_output.to(dtype=far_field.real.dtype).contiguous()
+        grad_output = grad_output.to(dtype=torch.float32 if ctx.native else far_field.real.dtype).contiguous()
         if ctx.native:
+            if ctx.recompute:
+                far_field, _ = native_forward(first, probe, positions_xy)
             grad_sample, grad_probe = native_backward(grad_output, far_field, first, probe, positions_xy)
         else:
             grad_sample, grad_probe = torch_backward(

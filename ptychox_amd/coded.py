@@ -15,7 +15,7 @@ import torch
 import torch.nn.functional as F
 
 from . import _lib
-from ._utils import can_use_native, require_native_frame
+from ._utils import can_use_native, require_native_frame, should_recompute
 
 
 def ptychoX_coded(
@@ -150,7 +150,14 @@ class PtychoXCodedFunction(torch.autograd.Function):
         spec, x_fwd, sensor, amp = native_forward(
             object_field, coded_surface, transfer_1, transfer_2, shifts_xy, downsample
         )
-        ctx.save_for_backward(spec, x_fwd, sensor, coded_surface, transfer_1, transfer_2, shifts_xy)
+        # x_forward and sensor_field are the per-translation tensors the
+        # reference keeps (python/ptychoX/coded/ops.py:108); recomputed from the
+        # inputs instead when they do not fit
+        ctx.recompute = should_recompute((x_fwd.numel() + sensor.numel()) * 8, object_field.device)
+        if ctx.recompute:
+            ctx.save_for_backward(object_field, None, None, coded_surface, transfer_1, transfer_2, shifts_xy)
+        else:
+            ctx.save_for_backward(spec, x_fwd, sensor, coded_surface, transfer_1, transfer_2, shifts_xy)
         ctx.downsample = int(downsample)
         return amp
 
@@ -158,6 +165,8 @@ class PtychoXCodedFunction(torch.autograd.Function):
     def backward(ctx, grad_output):  # type: ignore[override]
         spec, x_fwd, sensor, coded_surface, t1, t2, shifts_xy = ctx.saved_tensors
         grad_output = grad_output.to(torch.float32).contiguous()
+        if ctx.recompute:  # spec holds the object itself
+            spec, x_fwd, sensor, _ = native_forward(spec, coded_surface, t1, t2, shifts_xy, ctx.downsample)
         go, gc = native_backward(grad_output, spec, x_fwd, sensor, coded_surface, t1, t2, shifts_xy, ctx.downsample)
         return go, gc, None, None, None, None, None
 

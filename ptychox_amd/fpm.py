@@ -11,7 +11,7 @@ from __future__ import annotations
 import torch
 
 from . import _lib
-from ._utils import can_use_native, check_complex_scan_inputs, require_native_frame, scatter_flags
+from ._utils import can_use_native, check_complex_scan_inputs, require_native_frame, scatter_flags, should_recompute
 
 
 def ptychoX_fourier(
@@ -124,7 +124,14 @@ class PtychoXFPMFunction(torch.autograd.Function):
             if pupil.shape[0] > object_field.shape[0] or pupil.shape[1] > object_field.shape[1]:
                 raise ValueError("pupil must not be larger than object_field")
             object_fft, image_field, amplitude = native_forward(object_field, pupil, led)
-            ctx.save_for_backward(object_fft, pupil, image_field, led)
+            # the image fields are the per-LED tensor kept for the backward
+            # (python/ptychoX/fpm/ops.py:62 also keeps the sub-spectra, here
+            # re-gathered); recomputed from the object instead when they do not fit
+            ctx.recompute = should_recompute(image_field.numel() * 8, object_field.device)
+            if ctx.recompute:
+                ctx.save_for_backward(object_field, pupil, None, led)
+            else:
+                ctx.save_for_backward(object_fft, pupil, image_field, led)
             ctx.native = True
         else:
             sub, image_field, amplitude = torch_forward(object_field, pupil, led)
@@ -135,8 +142,10 @@ class PtychoXFPMFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_output):  # type: ignore[override]
         first, pupil, image_field, led = ctx.saved_tensors
-        grad_output = grad_output.to(dtype=image_field.real.dtype).contiguous()
+        grad_output = grad_output.to(dtype=torch.float32 if ctx.native else image_field.real.dtype).contiguous()
         if ctx.native:
+            if ctx.recompute:  # first is the object itself
+                first, image_field, _ = native_forward(first, pupil, led)
             grad_object, grad_pupil = native_backward(grad_output, image_field, first, pupil, led)
         else:
             grad_object, grad_pupil = torch_backward(grad_output, first, pupil, image_field, led, ctx.object_shape)

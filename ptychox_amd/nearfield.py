@@ -19,6 +19,7 @@ from ._utils import (
     can_use_native,
     check_complex_scan_inputs,
     check_windows,
+    should_recompute,
     scatter_flags,
     require_native_frame,
 )
@@ -130,17 +131,20 @@ class PtychoXNearFieldFunction(torch.autograd.Function):
     def forward(ctx, sample, probe, transfer, positions_xy, use_cuda_kernels=False):  # type: ignore[override]
         check_complex_scan_inputs(sample, probe, positions_xy)
         _check_transfer(transfer, probe)
+        h, w = probe.shape
+        # checked before the move to the device: CPU positions cost no synchronisation
+        check_windows(positions_xy, h, w, *sample.shape)
         positions_xy = positions_xy.to(device=sample.device, dtype=torch.long).contiguous()
         sample = sample.contiguous()
         probe = probe.contiguous()
         transfer = transfer.to(probe.dtype).contiguous()
-        h, w = probe.shape
         ctx.sample_shape = tuple(sample.shape)
-        check_windows(positions_xy, h, w, *sample.shape)
+        ctx.recompute = False
         if can_use_native(sample, probe, use_cuda_kernels):
             require_native_frame(h, w, "probe")
             field, amplitude = native_forward(sample, probe, transfer, positions_xy)
-            ctx.save_for_backward(sample, probe, transfer, field, positions_xy)
+            ctx.recompute = should_recompute(field.numel() * 8, sample.device)
+            ctx.save_for_backward(sample, probe, transfer, None if ctx.recompute else field, positions_xy)
             ctx.native = True
         else:
             patches, field, amplitude = torch_forward(sample, probe, transfer, positions_xy)
@@ -151,8 +155,10 @@ class PtychoXNearFieldFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_output):  # type: ignore[override]
         first, probe, transfer, field, positions_xy = ctx.saved_tensors
-        grad_output = grad_output.to(dtype=field.real.dtype).contiguous()
+        grad_output = grad_output.to(dtype=torch.float32 if ctx.native else field.real.dtype).contiguous()
         if ctx.native:
+            if ctx.recompute:
+                field, _ = native_forward(first, probe, transfer, positions_xy)
             gs, gp = native_backward(grad_output, field, first, probe, transfer, positions_xy)
         else:
             gs, gp = torch_backward(grad_output, first, probe, transfer, field, positions_xy, ctx.sample_shape)
