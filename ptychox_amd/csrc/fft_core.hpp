@@ -222,20 +222,32 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restr
 #pragma unroll
       for (int m = 1; m < R; ++m) a[m] = DIR > 0 ? c_mulc(a[m], reg_tw[i + m * NB]) : c_mul(a[m], reg_tw[i + m * NB]);
     } else if constexpr (NS > 1) {
-      // w^(k m) by recurrence from one table load: relative error grows by
-      // ~1 ulp per step (< 1e-6 for R = 16) and saves R - 2 memory loads
-      float2 w1;
-      if constexpr (LDS_TW) {
-        const float2 w = lds_tw[k * (N / (NS * R))];
-        w1 = DIR > 0 ? make_float2(w.x, -w.y) : w;
-      } else {
-        w1 = twiddle<DIR>(k * TWS);
-      }
-      float2 wm = w1;
+      // w^(k m): m = 1, 4 and 8 are table loads, w^2 = w w, every other power
+      // the product of the power of two below it and an earlier power
+      // (m = 7: w^4 w^3, w^3 = w^2 w).  A recurrence from w alone
+      // (w^(m+1) = w^m w) needs one load but multiplies the table's rounding
+      // by m and adds one rounding per step: twiddles 2x less accurate (rms
+      // 1.2e-7 vs 5.7e-8 at radix 8, 2.0e-7 vs 6.1e-8 at radix 16) and
+      // transforms ~1.5x less accurate than rocFFT at 512 points and above.
+      // Loading every power of two (w^2 too) gains little more accuracy and
+      // costs more in the passes that read the global table.
+      float2 w[R];
 #pragma unroll
       for (int m = 1; m < R; ++m) {
-        a[m] = c_mul(a[m], wm);
-        if (m + 1 < R) wm = c_mul(wm, w1);
+        if (m == 2) {
+          w[m] = c_mul(w[1], w[1]);
+        } else if ((m & (m - 1)) == 0) {
+          if constexpr (LDS_TW) {
+            const float2 t0 = lds_tw[k * m * (N / (NS * R))];
+            w[m] = DIR > 0 ? make_float2(t0.x, -t0.y) : t0;
+          } else {
+            w[m] = twiddle<DIR>(k * m * TWS);
+          }
+        } else {
+          const int hb = 1 << ilog2(m);
+          w[m] = c_mul(w[hb], w[m - hb]);
+        }
+        a[m] = c_mul(a[m], w[m]);
       }
     }
     dft<R, DIR>(a);
