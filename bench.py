@@ -36,8 +36,35 @@ OBJ, PROBE, BATCH = 2048, 512, 64
 # HBM bytes moved per launch, from rocprofv3 byte counters on this workload
 # (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a plain 256 MiB copy kernel
 # reads back; profiles/base_step_obj2048_probe512_b64_pmc_traffic.md).  The
-# counters need their own profiled runs, so the figure is recorded here.
-MEASURED_TRAFFIC = {"base step cols": 336_001_600}
+# counters need their own profiled runs, so the figure is recorded here with a
+# digest of the sources the kernel is compiled from: when they change, the
+# figure no longer describes the kernel and the bench reports null.
+MEASURED_TRAFFIC = {"base step cols": {"bytes": 336_001_600, "sources_sha256": None}}
+TRAFFIC_SOURCES = ("ptychox_amd/csrc/ptx_base.hip", "ptychox_amd/csrc/scan_common.hpp",
+                   "ptychox_amd/csrc/passes.hpp", "ptychox_amd/csrc/fft_core.hpp",
+                   "ptychox_amd/csrc/buffer_io.hpp", "ptychox_amd/csrc/common_kernels.hpp",
+                   "ptychox_amd/csrc/ptx_internal.hpp")
+
+
+def kernel_sources_sha256() -> str:
+    import hashlib
+
+    root = os.path.dirname(os.path.abspath(__file__))
+    h = hashlib.sha256()
+    for rel in TRAFFIC_SOURCES:
+        with open(os.path.join(root, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def recorded_traffic(kernel: str):
+    """(bytes, note) of the profiled traffic of `kernel`, or (None, why)."""
+    rec = MEASURED_TRAFFIC.get(kernel)
+    if rec is None:
+        return None, "not profiled"
+    if rec["sources_sha256"] != kernel_sources_sha256():
+        return None, "stale: the kernel's sources changed since the profile"
+    return rec["bytes"], "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/base_step_obj2048_probe512_b64_pmc_traffic.md"
 
 # Stated latency bound of the served mix: p99 from arrival to completion at
 # every measured Poisson rate up to 0.9x the server's saturation throughput.
@@ -54,7 +81,7 @@ def parse_args(argv=None):
     ap.add_argument("--no-plain", action="store_true", help="skip the unfused torch path")
     ap.add_argument("--atomic", action="store_true",
                     help="object gradient by float atomics instead of the (default) sorted-segment gather")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in FFT schedule (skip the autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB float32 scale-copy kernel (calibrates HBM byte counters)")
@@ -115,6 +142,105 @@ def _timed(fn, reps):
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def _host_cpu() -> tuple[str, int, int, int]:
+    """(CPU model, physical cores and logical CPUs this process may run on,
+    threads to use): every physical core, unless the environment caps the
+    host's share (OMP_NUM_THREADS, as the GPU hosts set it)."""
+    model, phys = "unknown", set()
+    allowed = os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else set(range(os.cpu_count() or 1))
+    try:
+        cpu, pkg = None, "0"
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "processor":
+                cpu = int(v)
+            elif k == "model name":
+                model = v
+            elif k == "physical id":
+                pkg = v
+            elif k == "core id" and cpu in allowed:
+                phys.add((pkg, v))
+    except OSError:
+        pass
+    n_phys = len(phys) or len(allowed)
+    threads = n_phys
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        threads = min(threads, int(cap))
+    return model, n_phys, len(allowed), threads
+
+
+def measure_cpu_baseline(ds, batches, seconds: float = 8.0):
+    """The reference's torch path restated literally (per-view Python loops,
+    oracle/torch_literal.py), FP32 on the host cores.
+
+    Headline workload: repetitions of one 16-position mini-batch of the
+    2048^2 / 512^2 workload, median of 5 timed groups.  The reference's
+    CPU-runnable configuration (object 512^2, probe 128^2, 225 positions,
+    mini-batch 32): fwd + MSE + adjoint over all positions (median of 5
+    epochs after a warm-up epoch) and the wall time of a 10-epoch Adam run
+    (median of 3)."""
+    import statistics
+
+    import torch
+
+    from oracle import torch_literal
+    from ptychox_amd.synthetic import conventional_dataset
+
+    model, n_phys, n_logical, threads = _host_cpu()
+    torch.set_num_threads(threads)
+    host = {"cpu_model": model, "physical_cores": n_phys, "logical_cpus": n_logical}
+    cs, cp = ds.sample.cpu(), ds.probe.cpu()
+    nsub = 16
+    cpos, cmeas = batches[0][0][:nsub].cpu(), batches[0][1][:nsub].cpu()
+    torch_literal.base_step(cs, cp, cpos, cmeas)  # warm-up
+    rates, t_all = [], 0.0
+    for _ in range(5):
+        reps, t = 0, 0.0
+        while t < seconds / 5 or reps < 1:
+            t1 = time.perf_counter()
+            torch_literal.base_step(cs, cp, cpos, cmeas)
+            t += time.perf_counter() - t1
+            reps += 1
+        rates.append(nsub * reps / t)
+        t_all += t
+    cpu = {"value": round(statistics.median(rates), 2), "unit": "patterns/s", "cores": threads, "kind": "port",
+           "sample": f"median of 5 groups ({t_all:.1f} s in all) of one {nsub}-position mini-batch of the same "
+                     f"workload (512x512 probe on the 2048x2048 object): fwd + MSE + adjoint, FP32 torch CPU, "
+                     f"per-view Python loops as in the reference", **host}
+    print(f"[bench] cpu baseline (headline workload): {cpu['value']} patterns/s on {threads} threads ({model})",
+          file=sys.stderr, flush=True)
+
+    small = conventional_dataset(512, 128, 32.0, 15, 24, 24, 2, 1e4, 4.0, "cpu", seed=0)
+    s0 = torch.rand((512, 512), generator=torch.Generator().manual_seed(0)).to(torch.complex64)
+    p0 = (small.probe.abs() > 0).to(torch.complex64)
+    P = small.positions.shape[0]
+    gen = torch.Generator().manual_seed(1)
+    torch_literal.base_epoch(s0, p0, small.positions, small.measured, 32, torch.randperm(P, generator=gen))
+    ep = []
+    for _ in range(5):
+        order = torch.randperm(P, generator=gen)
+        t1 = time.perf_counter()
+        torch_literal.base_epoch(s0, p0, small.positions, small.measured, 32, order)
+        ep.append(time.perf_counter() - t1)
+    walls = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        _, _, last_loss = torch_literal.base_train(s0, p0, small.positions, small.measured, 32, 10, 1e-2)
+        walls.append(time.perf_counter() - t1)
+    cpu_small = {"value": round(P / statistics.median(ep), 1), "unit": "patterns/s", "cores": threads,
+                 "kind": "port", "ten_epoch_wall_s": round(statistics.median(walls), 3),
+                 "ten_epoch_final_loss": round(last_loss, 3),
+                 "sample": "object 512x512, probe 128x128, 225 positions, mini-batch 32 (the reference's "
+                           "CPU-runnable configuration): fwd + MSE + adjoint over all positions, median of 5 "
+                           "epochs after one warm-up epoch; 10-epoch Adam run (lr 1e-2, cosine), median of 3",
+                 **host}
+    print(f"[bench] cpu baseline (512^2/128^2, 225 positions): {cpu_small['value']} patterns/s, 10 epochs "
+          f"{cpu_small['ten_epoch_wall_s']} s", file=sys.stderr, flush=True)
+    return cpu, cpu_small
 
 
 def measure_epoch(ds, dev, batch: int = BATCH, label: str = "900 positions") -> dict:
@@ -472,29 +598,9 @@ def main(argv=None) -> int:
         print(f"[bench] plain torch path: {pms:.3f} ms/step ({plain['value']} patterns/s)", file=sys.stderr, flush=True)
 
     # ---- CPU baseline: the reference torch path restated literally, host cores
-    cpu = None
+    cpu = cpu_small = None
     if not args.no_cpu:
-        from oracle import torch_literal
-
-        threads = min(16, os.cpu_count() or 1)
-        torch.set_num_threads(threads)
-        cs = ds.sample.cpu()
-        cp = ds.probe.cpu()
-        nsub = 16
-        cpos = batches[0][0][:nsub].cpu()
-        cmeas = batches[0][1][:nsub].cpu()
-        torch_literal.base_step(cs, cp, cpos, cmeas)  # warm-up
-        reps, t_cpu = 0, 0.0
-        while t_cpu < args.cpu_seconds or reps < 2:
-            t1 = time.perf_counter()
-            torch_literal.base_step(cs, cp, cpos, cmeas)
-            t_cpu += time.perf_counter() - t1
-            reps += 1
-        cpu = {"value": round(nsub * reps / t_cpu, 2), "unit": "patterns/s", "cores": threads, "kind": "port",
-               "sample": f"{reps} repetitions of one {nsub}-position mini-batch of the same workload "
-                         f"(512x512 probe on the 2048x2048 object), fwd + MSE + adjoint, FP32 torch CPU, "
-                         f"per-view Python loops as in the reference, {t_cpu:.1f} s"}
-        print(f"[bench] cpu baseline: {cpu['value']} patterns/s on {threads} threads", file=sys.stderr, flush=True)
+        cpu, cpu_small = measure_cpu_baseline(ds, batches, seconds=args.cpu_seconds)
 
     if args.copy_probe:
         # a plain streaming kernel of known size: reads 256 MiB, writes 256 MiB
@@ -514,6 +620,10 @@ def main(argv=None) -> int:
         small = _cds(512, 128, 32.0, 15, 24, 24, 2, 1e4, 4.0, dev, seed=0)
         epoch_small = measure_epoch(small, dev, batch=32, label="512^2 object, 128^2 probe, 225 positions")
         del small
+    if cpu_small is not None and epoch_small is not None:
+        g = epoch_small["graph"]["ms_per_epoch"]
+        cpu_small["gpu_graph_epoch_ms"] = g
+        cpu_small["gpu_speedup_10_epochs"] = round(cpu_small["ten_epoch_wall_s"] * 1e3 / (10 * g), 1)
     modalities = None if args.no_modalities else measure_modalities(dev, args)
     serving = None if args.no_serve else measure_serving(dev, args)
 
@@ -537,13 +647,15 @@ def main(argv=None) -> int:
                    "object_gradient": "atomic" if args.atomic else "sorted-segment"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get(dom), "algorithmic_bytes": int(dom_bytes)},
+                     "traffic": recorded_traffic(dom)[0], "traffic_source": recorded_traffic(dom)[1],
+                     "algorithmic_bytes": int(dom_bytes)},
         "path_roofline": {"bytes_per_pattern": b_alg, "achieved": round(path_gbs, 1), "unit": "GB/s",
                           "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
         "tuning": tuning,
         "plain_path": plain,
         "cpu_baseline": cpu,
+        "cpu_baseline_small_workload": cpu_small,
         "epoch": epoch,
         "epoch_small": epoch_small,
         "other_workloads": modalities,

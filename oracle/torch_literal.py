@@ -110,3 +110,45 @@ def nearfield_forward(sample, probe, transfer, positions_xy):
         patches.append(sample[y : y + h, x : x + w])
     stack = torch.stack(patches, dim=0)
     return torch.fft.ifft2(transfer[None] * torch.fft.fft2(stack * probe[None])).abs()
+
+
+def base_epoch(sample, probe, positions_xy, measured, batch, order):
+    """fwd + MSE + adjoint over every position in mini-batches of ``batch`` in
+    the given order (``python/train_ptychography.py:531-553``: ``randperm``,
+    slices of the permutation, a partial last batch); returns the gradients of
+    the last mini-batch."""
+    out = None
+    for start in range(0, positions_xy.shape[0], batch):
+        bi = order[start : start + batch]
+        out = base_step(sample, probe, positions_xy[bi], measured[bi])
+    return out
+
+
+def base_train(sample, probe, positions_xy, measured, batch, epochs, lr, seed=0):
+    """The reference driver's reconstruction loop on the CPU
+    (``python/train_ptychography.py:473-603``): per epoch a ``randperm`` of the
+    positions, mini-batches through the literal forward / MSE / adjoint, one
+    ``torch.optim.Adam`` step (betas (0.9, 0.999), eps 1e-16) per mini-batch on
+    object and probe, cosine learning rate (460-465) set at each epoch start.
+    Returns the final object, probe and the last epoch's mean loss."""
+    s = torch.nn.Parameter(sample.clone())
+    p = torch.nn.Parameter(probe.clone())
+    opt = torch.optim.Adam([s, p], lr=lr, betas=(0.9, 0.999), eps=1e-16)
+    gen = torch.Generator().manual_seed(seed)
+    P = positions_xy.shape[0]
+    last = 0.0
+    for epoch in range(1, epochs + 1):
+        phase = 1.0 if epochs <= 1 else (1.0 + math.cos(math.pi * (epoch - 1) / epochs)) * 0.5
+        for g in opt.param_groups:
+            g["lr"] = lr * phase
+        order = torch.randperm(P, generator=gen)
+        total = 0.0
+        for start in range(0, P, batch):
+            bi = order[start : start + batch]
+            with torch.no_grad():
+                loss, gs, gp = base_step(s.detach(), p.detach(), positions_xy[bi], measured[bi])
+            s.grad, p.grad = gs, gp
+            opt.step()
+            total += float(loss) * len(bi)
+        last = total / P
+    return s.detach(), p.detach(), last

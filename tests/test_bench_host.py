@@ -40,3 +40,22 @@ def test_no_gpu_is_a_clear_refusal(capsys, monkeypatch):
         return  # only meaningful on a host without a GPU
     assert bench.main(["--steps", "1", "--warmup", "1"]) == 2
     assert "needs a GPU" in capsys.readouterr().err
+
+
+def test_recorded_traffic_follows_the_kernel_sources(monkeypatch):
+    import bench
+
+    rec = dict(bench.MEASURED_TRAFFIC["base step cols"])
+    monkeypatch.setitem(bench.MEASURED_TRAFFIC, "base step cols", dict(rec, sources_sha256="0" * 64))
+    assert bench.recorded_traffic("base step cols")[0] is None  # sources changed: no figure
+    monkeypatch.setitem(bench.MEASURED_TRAFFIC, "base step cols",
+                        dict(rec, sources_sha256=bench.kernel_sources_sha256()))
+    assert bench.recorded_traffic("base step cols")[0] == rec["bytes"]
+    assert bench.recorded_traffic("no such kernel") == (None, "not profiled")
+
+
+def test_host_cpu_description():
+    import bench
+
+    model, n_phys, n_logical, threads = bench._host_cpu()
+    assert isinstance(model, str) and n_phys >= 1 and n_logical >= n_phys // 2 and 1 <= threads <= n_phys
