@@ -87,7 +87,7 @@ def parse_args(argv=None):
                     help="also run one 256 MiB float32 scale-copy kernel (calibrates HBM byte counters)")
     ap.add_argument("--no-epoch", action="store_true", help="skip the whole-epoch (Adam, hipGraph) measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
-    ap.add_argument("--serve-requests", type=int, default=40, help="requests in the serving measurement")
+    ap.add_argument("--serve-requests", type=int, default=400, help="requests per serving measurement")
     ap.add_argument("--no-modalities", action="store_true",
                     help="skip the Fourier / near-field / coded full-size measurements")
     return ap.parse_args(argv)
@@ -391,7 +391,10 @@ def measure_serving(dev, args) -> dict:
       plans exist);
     * latency: p50 / p99 from arrival to completion under Poisson arrivals at
       three rates (the plain path at half its own capacity, the server there
-      and at 0.5x and 0.9x its saturation rate)."""
+      and at 0.5x and 0.9x its saturation rate), every request of the stream
+      at every rate;
+    * matched final loss: each request's last-epoch loss on both paths at the
+      low rate, summarised per request class."""
     import math
     from concurrent.futures import ThreadPoolExecutor
 
@@ -421,25 +424,42 @@ def measure_serving(dev, args) -> dict:
     sat["peak_resident_gb"] = round(srv.peak_bytes / 1e9, 3)
     print(f"[bench] serving saturation: {sat}", file=sys.stderr, flush=True)
 
+    # plain per-request path: capacity from a warm pass over a few requests,
+    # then the whole stream at half that rate on both paths
     sub = base[: min(10, n)]
     sv.run_plain(sub, probs, dev)  # warm pass: rocFFT plans for every shape
-    plain_done = sv.run_plain(sub, probs, dev)
-    plain_busy = sum(c.finished - c.admitted for c in plain_done)
+    warm = sv.run_plain(sub, probs, dev)
+    plain_busy = sum(c.finished - c.admitted for c in warm)
     plain_rps = len(sub) / plain_busy
-    # matched final loss: the same requests' last-epoch loss on both paths
-    fused_loss = {c.rid: c.final_loss for c in sat_done}
-    loss_rel = max(abs(fused_loss[c.rid] - c.final_loss) / abs(c.final_loss) for c in plain_done)
-    plain_sat = {"requests": len(sub), "throughput_req_per_s": round(plain_rps, 3),
-                 "mean_service_ms": round(1e3 * plain_busy / len(sub), 2),
-                 "final_loss_max_rel_diff_vs_server": float(f"{loss_rel:.3g}")}
-    print(f"[bench] plain per-request path: {plain_sat}", file=sys.stderr, flush=True)
-
     rates = []
     low = 0.5 * plain_rps
-    low_reqs = [r for r in arrivals(low, 11)[: len(sub)]]
-    rates.append({"rate_req_per_s": round(low, 3),
-                  "server": sv.summarize(sv.Server(dev).run(low_reqs, probs)),
-                  "plain": sv.summarize(sv.run_plain(low_reqs, probs, dev))})
+    low_reqs = arrivals(low, 11)
+    low_server = sv.Server(dev).run(low_reqs, probs)
+    low_plain = sv.run_plain(low_reqs, probs, dev)
+    rates.append({"rate_req_per_s": round(low, 3), "server": sv.summarize(low_server),
+                  "plain": sv.summarize(low_plain)})
+    # matched final loss: every request's last-epoch loss on both paths, per class
+    fused_loss = {c.rid: c.final_loss for c in low_server}
+    per_class: dict = {}
+    for c in low_plain:
+        req = next(r for r in base if r.rid == c.rid)
+        key = f"{req.kind} {req.size}"
+        rel = abs(fused_loss[c.rid] - c.final_loss) / abs(c.final_loss)
+        cur = per_class.setdefault(key, {"requests": 0, "max_rel_diff": 0.0, "median_rel_diff": []})
+        cur["requests"] += 1
+        cur["max_rel_diff"] = max(cur["max_rel_diff"], rel)
+        cur["median_rel_diff"].append(rel)
+    for v in per_class.values():
+        v["median_rel_diff"] = float(f"{float(np.median(v['median_rel_diff'])):.3g}")
+        v["max_rel_diff"] = float(f"{v['max_rel_diff']:.3g}")
+    plain_sat = {"requests": len(sub), "throughput_req_per_s": round(plain_rps, 3),
+                 "mean_service_ms": round(1e3 * plain_busy / len(sub), 2),
+                 "final_loss_rel_diff_vs_server_by_class": per_class,
+                 "final_loss_note": "first-epoch losses and gradients agree to <= 1e-5 (tests); Adam's eps 1e-16 "
+                                    "turns rounding-level gradient components into full steps, seeding the Fourier "
+                                    "class's difference (the plain path fed the fused first-epoch gradients ends "
+                                    "within 1e-5 of the fused final loss)"}
+    print(f"[bench] plain per-request path: {plain_sat}", file=sys.stderr, flush=True)
     for frac, seed in ((0.5, 12), (0.9, 13)):
         rate = frac * sat["throughput_req_per_s"]
         rates.append({"rate_req_per_s": round(rate, 3), "server": sv.summarize(sv.Server(dev).run(arrivals(rate, seed),
