@@ -71,6 +71,15 @@ typedef enum ptx_status {
  * object gradient is the forward DFT of the conjugated spectrum sum,
  * kernels_nearf.cuh:186-240).  With transfer_1 = NULL this is that operator. */
 #define PTX_FLAG_MATLAB_NEARF 8u
+/* ptx_base_step / _indexed / _noalloc: the path of square frames.  Frames
+ * of side 32 run by default as one fused kernel per chunk of views with the
+ * whole frame in LDS; every other shape takes the row / column passes
+ * through the workspace.  PTX_FLAG_THREE_PASS selects the passes for 32^2 as
+ * well; PTX_FLAG_WHOLE_FRAME selects the whole-frame kernel for sides 32, 64
+ * and 128 (measured slower than the passes at 128^2 on MI355X).  Results
+ * agree to FP32 rounding. */
+#define PTX_FLAG_THREE_PASS 16u
+#define PTX_FLAG_WHOLE_FRAME 32u
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* ptx_last_error(void);

@@ -19,6 +19,8 @@ PTX_FLAG_DETERMINISTIC = 1
 PTX_FLAG_ACCUMULATE = 2
 PTX_FLAG_ATOMIC_SCATTER = 4
 PTX_FLAG_MATLAB_NEARF = 8
+PTX_FLAG_THREE_PASS = 16
+PTX_FLAG_WHOLE_FRAME = 32
 
 PTX_LOSS_L2 = 0
 PTX_LOSS_FD_ISO = 1

@@ -171,7 +171,8 @@ class BaseStep:
     the reference drivers (python/train_ptychography.py:555).
     """
 
-    def __init__(self, sample_shape, probe_shape, batch, device, scatter=None, keep_amplitude=False):
+    def __init__(self, sample_shape, probe_shape, batch, device, scatter=None, keep_amplitude=False,
+                 path=None):
         self.H, self.W = (int(s) for s in sample_shape)
         self.h, self.w = (int(s) for s in probe_shape)
         require_native_frame(self.h, self.w, "probe")
@@ -179,7 +180,12 @@ class BaseStep:
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
-        self.flags = scatter_flags(scatter)
+        # path: None (the library's choice: whole-frame kernel for 32^2 probes,
+        # row / column passes otherwise), "three_pass" or "whole_frame"
+        paths = {None: 0, "three_pass": _lib.PTX_FLAG_THREE_PASS, "whole_frame": _lib.PTX_FLAG_WHOLE_FRAME}
+        if path not in paths:
+            raise ValueError(f"path must be one of {list(paths)}, got {path!r}")
+        self.flags = scatter_flags(scatter) | paths[path]
         self.ws = _lib.workspace(self._ws_bytes(), self.device)
         self.loss = torch.empty(self.batch, dtype=torch.float32, device=self.device)
         self.amplitude = (

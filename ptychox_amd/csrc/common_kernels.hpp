@@ -147,11 +147,13 @@ struct TileLists {
   int64_t ntiles;  // 0: nothing to do
   int inside_only;
   __host__ __device__ int64_t blocks() const { return ntiles; }
-  __device__ __forceinline__ void run(int64_t tile, int* wave_tot /* 4 ints of LDS */) const {
+  // any work-group size that is a multiple of 64, up to 1024
+  __device__ __forceinline__ void run(int64_t tile, int* wave_tot /* blockDim.x / 64 ints of LDS */) const {
     const int64_t ty0 = (tile / tiles_x) * tile_h, tx0 = (tile % tiles_x) * kTileW;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nwaves = (int)(blockDim.x >> 6);
     int count = 0;
-    for (int64_t base = 0; base < batch; base += 256) {
+    for (int64_t base = 0; base < batch; base += blockDim.x) {
       const int64_t b = base + threadIdx.x;
       bool hit = false;
       if (b < batch) {
@@ -169,10 +171,32 @@ struct TileLists {
       int off = count;
       for (int k = 0; k < wave; ++k) off += wave_tot[k];
       if (hit) lists[tile * batch + off + before] = (int)b;
-      count += wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+      for (int k = 0; k < nwaves; ++k) count += wave_tot[k];
       __syncthreads();
     }
     if (threadIdx.x == 0) counts[tile] = count;
+  }
+  // The same list built by one wave alone (no LDS, no work-group barrier), so
+  // each wave of a work-group can build the list of a different tile.
+  __device__ __forceinline__ void run_wave(int64_t tile) const {
+    const int64_t ty0 = (tile / tiles_x) * tile_h, tx0 = (tile % tiles_x) * kTileW;
+    const int lane = threadIdx.x & 63;
+    int count = 0;
+    for (int64_t base = 0; base < batch; base += 64) {
+      const int64_t b = base + lane;
+      bool hit = false;
+      if (b < batch) {
+        const int64_t px = pos[2 * b], py = pos[2 * b + 1];
+        const int64_t oy = pmod(py + off_y, H), ox = pmod(px + off_x, W);
+        const bool ry = pmod(ty0 - oy, H) < h || pmod(oy - ty0, H) < tile_h;
+        const bool rx = pmod(tx0 - ox, W) < w || pmod(ox - tx0, W) < kTileW;
+        hit = ry && rx && (!inside_only || window_ok(px, py, h, w, H, W));
+      }
+      const unsigned long long m = __ballot(hit);
+      if (hit) lists[tile * batch + count + __popcll(m & ((1ull << lane) - 1ull))] = (int)b;
+      count += __popcll(m);
+    }
+    if (lane == 0) counts[tile] = count;
   }
 };
 static __global__ void __launch_bounds__(256) k_tile_lists(TileLists w) {

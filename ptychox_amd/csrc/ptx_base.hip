@@ -280,6 +280,212 @@ struct StepColsRun {
   }
 };
 
+// ---------------------------------------------------------------- whole-frame fused step (frames <= 128^2)
+// For N x N frames with N <= 128 the whole frame (N^2 complex, 128 KiB at
+// N = 128) fits in LDS, so one work-group runs a view's complete step --
+// gather, 2-D FFT, amplitude, loss, Gamma, 2-D inverse FFT, window
+// contribution and probe partial -- and the frame never goes to HBM between
+// the transforms (the three-pass path writes and reads it twice).  A
+// work-group has N^2/16 threads holding 16 elements each (1024 threads at
+// N = 128): thread (s, t) = (tid / T, tid % T), T = N/16, holds elements
+// t + q T of sequence s.  Two orientations:
+//   column orientation: sequence s is column s of the frame (element = row);
+//   row orientation:    sequence s is row s (element = column).
+// Global loads and stores in either orientation touch T x 8 contiguous bytes
+// per sequence per instruction (64 B at N = 128).  The frame buffer in LDS
+// (row stride lds_len<N>, the padded per-sequence layout of the FFT core)
+// serves as every transform's exchange region and as the transpose buffer:
+//   window * probe * sign loaded in column orientation -> column FFT ->
+//   transpose -> row FFT (far field, fftshift by the sign) -> |Phi|, loss,
+//   Gamma (measured amplitudes in row orientation) -> row iFFT -> transpose ->
+//   column iFFT -> chi = sign * (.) in column orientation -> contribution
+//   chi conj(probe) and probe partial chi conj(window).
+// The probe partial sum over the work-group's chunk of views is kept in the
+// work-group's own slice of gp_part (read-modify-write after the first view).  The loss of
+// each view is a fixed-order work-group sum.  Work-groups past the views'
+// build the tile lists of the sorted-segment gather, one tile per wave.
+template <int N>
+struct FrameCfg {
+  static constexpr int E = 16;
+  static constexpr int T = N / E;
+  static constexpr int THREADS = N * T;
+  static constexpr int LS = lds_len<N>();
+  static_assert(N >= 32 && N <= 128, "whole-frame step: N in [32, 128]");
+  static_assert((size_t)N * LS * sizeof(float2) <= 150 * 1024, "frame must fit in LDS");
+};
+
+template <int N>
+__global__ void __launch_bounds__(FrameCfg<N>::THREADS)
+    k_base_frame_step(const float2* __restrict__ sample, int64_t H, int64_t W, const float2* __restrict__ probe,
+                      const int64_t* __restrict__ pos, const float* __restrict__ measured,
+                      const int64_t* __restrict__ order, int64_t count, int64_t B, int chunk, int nchunks,
+                      float gscale, float loss_scale, float* __restrict__ loss_out, float* __restrict__ amp_out,
+                      float2* __restrict__ gobj, float2* __restrict__ contrib, float2* __restrict__ gp_part,
+                      TileLists lists) {
+  using Cf = FrameCfg<N>;
+  constexpr int E = Cf::E, T = Cf::T, LS = Cf::LS;
+  __shared__ float2 frame[N * LS];
+  __shared__ float red[Cf::THREADS / 64 + 1];
+  if ((int)blockIdx.x >= nchunks) {  // side work: tile lists of the gather, one tile per wave
+    const int64_t tile = ((int64_t)blockIdx.x - nchunks) * (Cf::THREADS / 64) + (threadIdx.x >> 6);
+    if (tile < lists.ntiles) lists.run_wave(tile);
+    return;
+  }
+  const int s = (int)(threadIdx.x / T);  // sequence (column, then row)
+  const int t = (int)(threadIdx.x % T);
+  const int64_t b0 = (int64_t)blockIdx.x * chunk;
+  const int64_t b1 = b0 + chunk < B ? b0 + chunk : B;
+  constexpr uint32_t NN = (uint32_t)N * N;
+  // Buffer addressing: one 32-bit lane offset per access pattern, the element
+  // index's q term uniform (the sample below 2 GiB, checked by the host).
+  const uint32_t lane_c = (uint32_t)(t * N + s);  // column orientation, frame layout (row j = t + q T, column s)
+  const uint32_t lane_r = (uint32_t)(s * N + t);  // row orientation (row s, column t + q T)
+  const uint32_t obj_bytes = (uint32_t)(H * W * 8);
+  // the probe is read from L2 where it is needed (column orientation,
+  // fftshift sign folded in: p[j][s] (-1)^(j+s), j = t + q T); keeping it in
+  // registers for the chunk would exceed the 128-register budget of a
+  // 1024-thread work-group
+  const auto rp = buffer_rsrc(probe, NN * 8u);
+  auto probe_signed = [&](int q) {
+    const float2 p = buf_load_f2(rp, lane_c * 8u, (uint32_t)(q * T * N) * 8u);
+    return ((t + q * T + s) & 1) ? c_neg(p) : p;
+  };
+  const auto rgp = buffer_rsrc(gp_part + (int64_t)blockIdx.x * NN, NN * 8u);
+  bool first = true;  // no view of the chunk has written the probe slice yet
+  for (int64_t b = b0; b < b1; ++b) {
+    const int64_t x0 = pos[2 * b], y0 = pos[2 * b + 1];
+    const bool ok = window_ok(x0, y0, N, N, H, W);
+    int64_t mframe = b;
+    if (order) {
+      const uint64_t o = (uint64_t)order[b];
+      mframe = ((o >> 32) == 0 && (uint32_t)o < (uint32_t)count) ? (int64_t)o : 0;
+    }
+    // window * probe * sign, column orientation (an out-of-range window reads zero)
+    const auto rs = buffer_rsrc(sample, ok ? obj_bytes : 0u);
+    const uint32_t wlane = ok ? (uint32_t)(((y0 + t) * W + x0 + s) * 8) : 0u;
+    float2 v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = c_mul(buf_load_f2(rs, wlane, (uint32_t)(q * T * W * 8)), probe_signed(q));
+    lds_barrier();  // the previous view's last reads of the frame buffer are done
+    fft_seq<N, E, -1>(v, t, &frame[s * LS]);  // column FFT
+    lds_barrier();
+#pragma unroll
+    for (int q = 0; q < E; ++q) frame[(t + q * T) * LS + lds_pad(s)] = v[q];  // transpose
+    lds_barrier();
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = frame[s * LS + lds_pad(t + q * T)];  // row s
+    lds_barrier();
+    fft_seq<N, E, -1>(v, t, &frame[s * LS]);  // row FFT: Phi[s][t + q T]
+    const auto rm = buffer_rsrc(measured + (int64_t)mframe * NN, NN * 4u);
+    // amplitudes (when requested) through a zero-length buffer otherwise: no branch around the stores
+    const auto ra = buffer_rsrc(amp_out ? amp_out + (int64_t)b * NN : amp_out, amp_out ? NN * 4u : 0u);
+    float lsum = 0.0f;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const float y = buf_load_f1(rm, lane_r * 4u, (uint32_t)(q * T) * 4u);
+      const AbsInv ai = c_abs_inv(v[q], 1.0f / kFltEps);
+      const float d = ai.a - y;
+      lsum += d * d;
+      v[q] = c_scale(v[q], gscale * d * ai.inv);
+      buf_store_f1(ai.a, ra, lane_r * 4u, (uint32_t)(q * T) * 4u);
+    }
+    // loss of the view: fixed-order work-group sum
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      if (off < Cf::THREADS) lsum += __shfl_xor(lsum, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lsum;
+    lds_barrier();
+    fft_seq<N, E, +1>(v, t, &frame[s * LS]);  // row iFFT
+    if (threadIdx.x == 0) {
+      float tot = 0.0f;
+      for (int w = 0; w < Cf::THREADS / 64; ++w) tot += red[w];
+      loss_out[b] = tot * loss_scale;
+    }
+    lds_barrier();
+#pragma unroll
+    for (int q = 0; q < E; ++q) frame[(t + q * T) * LS + lds_pad(s)] = v[q];  // transpose
+    lds_barrier();
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = frame[s * LS + lds_pad(t + q * T)];  // column s
+    lds_barrier();
+    fft_seq<N, E, +1>(v, t, &frame[s * LS]);  // column iFFT: (-1)^(j+s) chi[j][s]
+    if (!ok) continue;
+    // contributions and probe partial, column orientation; the sign of chi is
+    // the sign folded into pr, so chi conj(p) = v conj(pr)
+    const auto rc = buffer_rsrc(contrib ? contrib + (int64_t)b * NN : contrib, contrib ? NN * 8u : 0u);
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int j = t + q * T;
+      const uint32_t qoff = (uint32_t)(q * T * N) * 8u;
+      const float2 val = c_mulc(v[q], probe_signed(q));
+      if (contrib) {
+        buf_store_f2(val, rc, lane_c * 8u, qoff);
+      } else {
+        float2* g = gobj + (y0 + j) * W + x0 + s;
+        unsafeAtomicAdd(&g->x, val.x);
+        unsafeAtomicAdd(&g->y, val.y);
+      }
+      const float2 chi = ((j + s) & 1) ? c_neg(v[q]) : v[q];
+      const float2 pp = c_mulc(chi, buf_load_f2(rs, wlane, (uint32_t)(q * T * W * 8)));
+      // each thread owns its elements of the chunk's probe slice
+      buf_store_f2(first ? pp : c_add(buf_load_f2(rgp, lane_c * 8u, qoff), pp), rgp, lane_c * 8u, qoff);
+    }
+    first = false;
+  }
+  if (first) {  // every view of the chunk was out of range: the slice is zero
+#pragma unroll
+    for (int q = 0; q < E; ++q) buf_store_f2(c_zero(), rgp, lane_c * 8u, (uint32_t)(q * T * N) * 8u);
+  }
+}
+
+// Whole-frame step: the fused kernel (with the tile lists riding along), then
+// the object-gradient gather with the probe-partial sum riding along, or the
+// atomic scatter's zero fill before and the partial sum after.
+template <int N>
+ptx_status frame_step_launch(const ScanWorkspace& ws, const ScatterPlan& sp, const float2* sample, int64_t H,
+                             int64_t W, const float2* probe, const int64_t* pos, const float* measured,
+                             const int64_t* order, int64_t count, int64_t B, float loss_scale, float* loss,
+                             float* amp, float2* gs, float2* gp, unsigned flags, hipStream_t s) {
+  using Cf = FrameCfg<N>;
+  const bool det = use_segments(flags);
+  const bool acc = flags & PTX_FLAG_ACCUMULATE;
+  ptx_status st;
+  if (!det && !acc) {
+    fill_zero2(gs, H * W, s);
+    if ((st = check_launch("zero fill", s)) != PTX_OK) return st;
+  }
+  TileLists lists{};
+  if (det) lists = gather_lists_work(ws.gather, pos, B, N, N, H, W, 0, 0, true);
+  constexpr int kWaves = Cf::THREADS / 64;
+  const int64_t side = (lists.ntiles + kWaves - 1) / kWaves;
+  k_base_frame_step<N><<<dim3((unsigned)(sp.nchunks + side)), dim3(Cf::THREADS), 0, s>>>(
+      sample, H, W, probe, pos, measured, order, count, B, sp.chunk, sp.nchunks, 2.0f * loss_scale,
+      1.0f / (float)(N * N), loss, amp, gs, det ? ws.tb : nullptr, ws.gp_part, lists);
+  if ((st = check_launch("base frame step", s)) != PTX_OK) return st;
+  const SumParts2 probe_sum{ws.gp_part, sp.nchunks, (int64_t)N * N, gp, acc ? 1 : 0, 0};
+  if (!det) {
+    sum_parts2(ws.gp_part, sp.nchunks, (int64_t)N * N, gp, acc ? 1 : 0, s);
+    return check_launch("probe partial sum", s);
+  }
+  SideWork with_gather{};
+  with_gather.sums = probe_sum;
+  return run_tile_gather(ws.gather, ws.tb, N, N, pos, B, 0, 0, gs, H, W, acc ? 1 : 0, s, with_gather);
+}
+
+// Which path a square frame takes.  The whole-frame kernel serialises a
+// view's work on one compute unit; measured against the three passes on
+// MI355X (objects 4x the probe side, batches 32..1024) it is 4-8 % faster at
+// 32^2, within -12..+7 % at 64^2 and 1.5-2.3x slower at 128^2 (there the
+// frame's 16 values per thread exceed the 128-register budget of a
+// 1024-thread work-group and spill), so it is the default at 32^2 only.
+// PTX_FLAG_WHOLE_FRAME forces it for 32..128, PTX_FLAG_THREE_PASS forbids it.
+inline bool frame_step_eligible(int64_t H, int64_t W, int64_t h, int64_t w, unsigned flags) {
+  if (flags & PTX_FLAG_THREE_PASS) return false;
+  if (h != w || H * W * 8 >= (int64_t(1) << 31)) return false;  // 32-bit buffer offsets into the sample
+  if (flags & PTX_FLAG_WHOLE_FRAME) return h == 32 || h == 64 || h == 128;
+  return h == 32;
+}
+
 }  // namespace
 }  // namespace ptx
 
@@ -406,6 +612,23 @@ ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t
     fill_zero2(reinterpret_cast<float2*>(ws.bad_index), 1, s);
     if ((st = launch_gather_positions(order, pos, count, B, ws.pos_batch, ws.bad_index, s)) != PTX_OK) return st;
     bpos = ws.pos_batch;
+  }
+  if (frame_step_eligible(H, W, h, w, flags)) {
+    auto* smp = (const float2*)sample;
+    auto* prb = (const float2*)probe;
+    auto* gsp = (float2*)grad_sample;
+    auto* gpp = (float2*)grad_probe;
+    switch (h) {
+      case 32:
+        return frame_step_launch<32>(ws, sp, smp, H, W, prb, bpos, measured, order, count, B, loss_scale,
+                                     loss_per_position, amplitude, gsp, gpp, flags, s);
+      case 64:
+        return frame_step_launch<64>(ws, sp, smp, H, W, prb, bpos, measured, order, count, B, loss_scale,
+                                     loss_per_position, amplitude, gsp, gpp, flags, s);
+      default:
+        return frame_step_launch<128>(ws, sp, smp, H, W, prb, bpos, measured, order, count, B, loss_scale,
+                                      loss_per_position, amplitude, gsp, gpp, flags, s);
+    }
   }
   ScanGatherRows<true> r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, bpos, ws.tb};
   st = dispatch_ne<GatherRowsRun>(PASS_ROW, w, r1, B * h, s);
