@@ -247,7 +247,9 @@ def measure_epoch(ds, dev, batch: int = BATCH, label: str = "900 positions") -> 
     """Whole training epochs of the reference driver's loop (permutation, all
     positions in mini-batches of `batch` with a partial last batch, MSE loss,
     Adam on object and probe): the device-resident loop captured as a hipGraph,
-    the same loop launched eagerly, and the plain torch loop."""
+    the same loop launched eagerly, the graph with the Adam updates as separate
+    launches instead of fused into the gradient epilogues, and the plain torch
+    loop."""
     import torch
 
     from ptychox_amd import ptychoX_base
@@ -257,10 +259,10 @@ def measure_epoch(ds, dev, batch: int = BATCH, label: str = "900 positions") -> 
     init_s = torch.ones_like(ds.sample)
     init_p = (ds.probe.abs() > 0).to(torch.complex64)
     out = {"positions": P, "mini_batch": batch}
-    for name, graph in (("graph", True), ("eager", False)):
+    for name, graph, fused in (("graph", True, True), ("eager", False, True), ("graph_unfused_adam", True, False)):
         gen = torch.Generator(device=dev).manual_seed(0)
         rec = ConventionalReconstruction(init_s, init_p, ds.positions, ds.measured, batch, lr=1e-2, epochs=10,
-                                         generator=gen, graph=graph)
+                                         generator=gen, graph=graph, fused_adam=fused)
         rec.run_epoch(1)
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
@@ -298,7 +300,8 @@ def measure_epoch(ds, dev, batch: int = BATCH, label: str = "900 positions") -> 
     out["plain_torch"] = {"ms_per_epoch": round(ms, 3), "patterns_per_s": round(P / (ms / 1e3), 1)}
     out["graph_speedup_vs_plain"] = round(ms / out["graph"]["ms_per_epoch"], 2)
     print(f"[bench] epoch ({label}, Adam): graph {out['graph']['ms_per_epoch']} ms, eager "
-          f"{out['eager']['ms_per_epoch']} ms, plain torch {ms:.3f} ms", file=sys.stderr, flush=True)
+          f"{out['eager']['ms_per_epoch']} ms, graph with separate Adam launches "
+          f"{out['graph_unfused_adam']['ms_per_epoch']} ms, plain torch {ms:.3f} ms", file=sys.stderr, flush=True)
     return out
 
 

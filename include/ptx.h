@@ -242,6 +242,37 @@ ptx_status ptx_base_step_noalloc(const void* sample, int64_t sample_h, int64_t s
 ptx_status ptx_adam_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
                          float beta1, float beta2, const float* hyper, unsigned flags, void* stream);
 
+/* Adam state of one complex64 unknown for ptx_base_train_step: moments of
+ * 2 floats per element and the device step-size table of ptx_adam_step. */
+typedef struct ptx_adam {
+  float* exp_avg;
+  float* exp_avg_sq;
+  const float* hyper; /* device float[3] = {lr / (1 - beta1^t), sqrt(1 - beta2^t), eps} */
+  float beta1, beta2;
+} ptx_adam;
+
+/* One mini-batch of the conventional driver's training loop
+ * (python/train_ptychography.py:549-559: ptychoX_base -> F.mse_loss ->
+ * backward -> optimizer.step()) as ptx_base_step_indexed followed by
+ * ptx_adam_step on the sample and on the probe, with the Adam updates applied
+ * where the gradients are finished (the object-gradient gather's and the
+ * probe-partial sum's epilogues) instead of storing the gradients and reading
+ * them back.  Results equal those of that sequence of calls.  sample and probe
+ * are updated in place.  adam_sample / adam_probe NULL: that unknown is not
+ * updated and its gradient is written to grad_sample / grad_probe as by
+ * ptx_base_step_indexed; otherwise that buffer is scratch of the same shape
+ * whose contents afterwards are unspecified.  order NULL: the batch is the
+ * tables' first `batch` entries.  PTX_FLAG_ACCUMULATE is rejected when an
+ * update is requested.  Workspace: ptx_base_step_workspace. */
+ptx_status ptx_base_train_step(void* sample, int64_t sample_h, int64_t sample_w,
+                               void* probe, int64_t probe_h, int64_t probe_w,
+                               const int64_t* positions_all, const float* measured_all, int64_t count,
+                               const int64_t* order, int64_t batch, float loss_scale,
+                               float* loss_per_position, float* amplitude,
+                               void* grad_sample, void* grad_probe,
+                               const ptx_adam* adam_sample, const ptx_adam* adam_probe,
+                               void* workspace, size_t workspace_bytes, unsigned flags, void* stream);
+
 /* positions_out[b] = positions_all[order[b]] (int64 pairs); entries of order
  * outside [0, count) read entry 0 and set *bad_index_flag to 1 (the caller
  * zeroes the flag).  An empty table (count 0) with batch > 0 is

@@ -80,6 +80,10 @@ SIGNATURES: dict[str, tuple] = {
         [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _f, _p, _p, _p, _p, _u, _p],
     ),
     "ptx_adam_step": (_st, [_p, _p, _p, _p, _i64, _f, _f, _p, _u, _p]),
+    "ptx_base_train_step": (
+        _st,
+        [_p, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _p, _i64, _f, _p, _p, _p, _p, _p, _p, _p, _sz, _u, _p],
+    ),
     "ptx_loss_workspace": (_sz, [_i64, _i64, _i64]),
     "ptx_amplitude_loss": (_st, [_i, _p, _p, _i64, _i64, _i64, _f, _f, _p, _p, _p, _sz, _p]),
     "ptx_field_regularizer": (_st, [_i, _p, _i64, _i64, _i64, _f, _f, _p, _p, _p, _sz, _p]),
@@ -117,6 +121,14 @@ SIGNATURES: dict[str, tuple] = {
         [_p, _i64, _i64, _p, _p, _p, _p, _i64, _i64, _p, _f, _p, _p, _p, _p, _p, _sz, _u, _p],
     ),
 }
+
+
+
+class PtxAdam(ctypes.Structure):
+    """``ptx_adam`` of include/ptx.h: one unknown's Adam moments and step-size table."""
+
+    _fields_ = [("exp_avg", _p), ("exp_avg_sq", _p), ("hyper", _p), ("beta1", _f), ("beta2", _f)]
+
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None

@@ -23,6 +23,49 @@ inline void fill_zero2(float2* p, int64_t n, hipStream_t s) {
   if (n > 0) k_fill_zero2<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(p, n);
 }
 
+// One Adam update of a complex element, per real component, with
+// torch.optim.Adam's foreach arithmetic as the drivers configure it
+// (python/train_ptychography.py:500-507):
+//   m = lerp(m, g, 1 - b1);  v = b2 v + (1 - b2) g^2
+//   p += -(lr / (1 - b1^t)) * m / (sqrt(v) / sqrt(1 - b2^t) + eps)
+// hyper: device float[3] = {lr / (1 - b1^t), sqrt(1 - b2^t), eps}.  Used by
+// the stand-alone Adam kernel and by the gradient epilogues that fuse the
+// update (param == null: no update, the epilogue stores the gradient).
+// Every rounding is spelled out (contraction off, the fused multiply-adds
+// written as such), so the update is bitwise the same in every kernel that
+// inlines it: one fma per foreach op (lerp, addcmul after the scale by b2,
+// addcdiv), the square root, the divisions and the eps add rounded separately.
+__device__ __forceinline__ float adam_component(float p, float g, float& m, float& v, float b1, float b2,
+                                                float step, float bc2, float eps) {
+#pragma clang fp contract(off)
+  const float mi = __builtin_fmaf(1.0f - b1, g - m, m);        // lerp(m, g, 1 - b1), weight < 0.5
+  const float vi = __builtin_fmaf((1.0f - b2) * g, g, v * b2);  // v * b2, then addcmul(g, g, 1 - b2)
+  m = mi;
+  v = vi;
+  const float denom = sqrtf(vi) / bc2 + eps;
+  return __builtin_fmaf(-step, mi / denom, p);  // addcdiv(m, denom, -step)
+}
+
+struct AdamArgs {
+  float2* param;  // complex parameter, updated in place
+  float* m;       // first moments, 2 floats per element
+  float* v;       // second moments
+  const float* hyper;
+  float beta1, beta2;
+  __device__ __forceinline__ void apply(int64_t i, float2 g) const {
+    const float step = hyper[0], bc2 = hyper[1], eps = hyper[2];
+    float2 p = param[i];
+    float m0 = m[2 * i], m1 = m[2 * i + 1], v0 = v[2 * i], v1 = v[2 * i + 1];
+    p.x = adam_component(p.x, g.x, m0, v0, beta1, beta2, step, bc2, eps);
+    p.y = adam_component(p.y, g.y, m1, v1, beta1, beta2, step, bc2, eps);
+    param[i] = p;
+    m[2 * i] = m0;
+    m[2 * i + 1] = m1;
+    v[2 * i] = v0;
+    v[2 * i + 1] = v1;
+  }
+};
+
 // out[i] (+)= sum_{c < nparts} part[c*n + i], summed in ascending c (deterministic)
 // conj: out (+)= conj(sum of the parts).  One 256-thread work-group per 256 outputs.
 struct SumParts2 {
@@ -30,6 +73,7 @@ struct SumParts2 {
   int64_t nparts, n;
   float2* out;
   int accumulate, conj;
+  AdamArgs adam{};  // adam.param set: the sum is the gradient of an Adam update (out unused)
   __host__ __device__ int64_t blocks() const { return n > 0 ? (n + 255) / 256 : 0; }
   __device__ __forceinline__ void run(int64_t blk) const {
     const int64_t i = blk * 256 + threadIdx.x;
@@ -47,7 +91,9 @@ struct SumParts2 {
       for (; c < nparts; ++c) s = c_add(s, part[c * n + i]);
       return s;
     };
-    if (!conj) {
+    if (adam.param) {
+      adam.apply(i, sum_into(c_zero()));
+    } else if (!conj) {
       out[i] = sum_into(accumulate ? out[i] : c_zero());
     } else {
       float2 t = sum_into(c_zero());
@@ -319,7 +365,8 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
                                                             const int* __restrict__ n_items,
                                                             float2* __restrict__ partial,
                                                             float2* __restrict__ grad, int64_t H, int64_t W,
-                                                            int accumulate, SideWork side, int64_t main_blocks) {
+                                                            int accumulate, SideWork side, int64_t main_blocks,
+                                                            AdamArgs adam) {
   constexpr int TH = SEG ? kSegTileH : kFlatTileH;
   constexpr int RPT = TH * kTileW / 256;  // rows per thread
   constexpr int RSTEP = 256 / kTileW;
@@ -402,12 +449,20 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
     return;
   }
   if (!xin) return;
+  // the tile's final gradient (single pass, or a segmented tile with one
+  // segment) feeds the fused Adam update; otherwise it is stored (and the
+  // combine pass finishes segmented tiles)
+  const bool final_here = !SEG || n <= kSegViews;
 #pragma unroll
   for (int j = 0; j < RPT; ++j) {
     const int64_t Y = Y0 + j * RSTEP;
     if (Y < H) {
-      float2* d = grad + Y * W + X;
-      *d = accumulate ? c_add(*d, acc[j]) : acc[j];
+      if (adam.param && final_here) {
+        adam.apply(Y * W + X, acc[j]);
+      } else {
+        float2* d = grad + Y * W + X;
+        *d = accumulate ? c_add(*d, acc[j]) : acc[j];
+      }
     }
   }
 }
@@ -419,7 +474,8 @@ constexpr int kCombineBatch = 8;
 static __global__ void __launch_bounds__(256) k_gather_combine(const int* __restrict__ counts,
                                                                const int* __restrict__ slot_off, int64_t tiles_x,
                                                                const float2* __restrict__ partial,
-                                                               float2* __restrict__ grad, int64_t H, int64_t W) {
+                                                               float2* __restrict__ grad, int64_t H, int64_t W,
+                                                               AdamArgs adam) {
   constexpr int RPT = kSegTilePx / 256;
   constexpr int RSTEP = 256 / kTileW;
   const int64_t tile = blockIdx.x / RPT;
@@ -441,7 +497,10 @@ static __global__ void __launch_bounds__(256) k_gather_combine(const int* __rest
     for (int k = 0; k < kCombineBatch; ++k)
       if (sg + k < nseg - 1) s = c_add(s, v[k]);
   }
-  grad[Y * W + X] = s;
+  if (adam.param)
+    adam.apply(Y * W + X, s);
+  else
+    grad[Y * W + X] = s;
 }
 
 struct TileGrid {
@@ -519,16 +578,18 @@ inline TileLists gather_lists_work(const GatherScratch& g, const int64_t* pos, i
 // launch built (gather_lists_work as its side work): the single-pass gather,
 // or plan, segmented gather and combine.  `side` rides along with the gather
 // launch.
+// adam.param set: the object gradient feeds that Adam update instead of
+// being stored (grad is then scratch of segmented tiles; no accumulate).
 inline ptx_status run_tile_gather(const GatherScratch& g, const float2* contrib, int64_t h, int64_t w,
                                   const int64_t* pos, int64_t B, int64_t off_y, int64_t off_x, float2* grad,
                                   int64_t H, int64_t W, int accumulate, hipStream_t s,
-                                  const SideWork& side = SideWork{}) {
+                                  const SideWork& side = SideWork{}, const AdamArgs& adam = AdamArgs{}) {
   const TileGrid tg = tile_grid(h, w, B, H, W);
   ptx_status st;
   if (!gather_segmented(h, w, B, H, W)) {
     k_tile_gather<false><<<dim3((unsigned)(tg.count() + side.blocks())), dim3(256), 0, s>>>(
         contrib, (int)h, (int)w, pos, B, off_y, off_x, g.lists, g.counts, tg.tiles_x, nullptr, nullptr, nullptr,
-        nullptr, nullptr, grad, H, W, accumulate, side, tg.count());
+        nullptr, nullptr, grad, H, W, accumulate, side, tg.count(), adam);
     return check_launch("tile gather", s);
   }
   const GatherBounds gb = gather_bounds(tg.count(), h, w, B);
@@ -536,12 +597,12 @@ inline ptx_status run_tile_gather(const GatherScratch& g, const float2* contrib,
   if ((st = check_launch("gather plan", s)) != PTX_OK) return st;
   k_tile_gather<true><<<dim3((unsigned)(gb.items + side.blocks())), dim3(256), 0, s>>>(
       contrib, (int)h, (int)w, pos, B, off_y, off_x, g.lists, g.counts, tg.tiles_x, g.seg_off, g.slot_off,
-      g.item_tile, g.n_items, g.partial, grad, H, W, accumulate, side, gb.items);
+      g.item_tile, g.n_items, g.partial, grad, H, W, accumulate, side, gb.items, adam);
   if ((st = check_launch("tile gather", s)) != PTX_OK) return st;
   if (gb.slots > 0) {
     k_gather_combine<<<dim3((unsigned)(tg.count() * (kSegTilePx / 256))), dim3(256), 0, s>>>(g.counts, g.slot_off,
                                                                                           tg.tiles_x, g.partial,
-                                                                                          grad, H, W);
+                                                                                          grad, H, W, adam);
     if ((st = check_launch("gather combine", s)) != PTX_OK) return st;
   }
   return PTX_OK;

@@ -445,7 +445,8 @@ template <int N>
 ptx_status frame_step_launch(const ScanWorkspace& ws, const ScatterPlan& sp, const float2* sample, int64_t H,
                              int64_t W, const float2* probe, const int64_t* pos, const float* measured,
                              const int64_t* order, int64_t count, int64_t B, float loss_scale, float* loss,
-                             float* amp, float2* gs, float2* gp, unsigned flags, hipStream_t s) {
+                             float* amp, float2* gs, float2* gp, unsigned flags, hipStream_t s,
+                             const AdamArgs& adam_obj, const AdamArgs& adam_probe) {
   using Cf = FrameCfg<N>;
   const bool det = use_segments(flags);
   const bool acc = flags & PTX_FLAG_ACCUMULATE;
@@ -462,14 +463,17 @@ ptx_status frame_step_launch(const ScanWorkspace& ws, const ScatterPlan& sp, con
       sample, H, W, probe, pos, measured, order, count, B, sp.chunk, sp.nchunks, 2.0f * loss_scale,
       1.0f / (float)(N * N), loss, amp, gs, det ? ws.tb : nullptr, ws.gp_part, lists);
   if ((st = check_launch("base frame step", s)) != PTX_OK) return st;
-  const SumParts2 probe_sum{ws.gp_part, sp.nchunks, (int64_t)N * N, gp, acc ? 1 : 0, 0};
+  SumParts2 probe_sum{ws.gp_part, sp.nchunks, (int64_t)N * N, gp, acc ? 1 : 0, 0};
+  probe_sum.adam = adam_probe;
   if (!det) {
-    sum_parts2(ws.gp_part, sp.nchunks, (int64_t)N * N, gp, acc ? 1 : 0, s);
-    return check_launch("probe partial sum", s);
+    if (probe_sum.blocks() > 0) k_sum_parts2<<<dim3((unsigned)probe_sum.blocks()), dim3(256), 0, s>>>(probe_sum);
+    if ((st = check_launch("probe partial sum", s)) != PTX_OK) return st;
+    if (adam_obj.param) return launch_adam(adam_obj, gs, H * W, s);  // after the atomic scatter
+    return PTX_OK;
   }
   SideWork with_gather{};
   with_gather.sums = probe_sum;
-  return run_tile_gather(ws.gather, ws.tb, N, N, pos, B, 0, 0, gs, H, W, acc ? 1 : 0, s, with_gather);
+  return run_tile_gather(ws.gather, ws.tb, N, N, pos, B, 0, 0, gs, H, W, acc ? 1 : 0, s, with_gather, adam_obj);
 }
 
 // Which path a square frame takes.  The whole-frame kernel serialises a
@@ -579,11 +583,18 @@ namespace ptx {
 ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t W, const void* probe, int64_t h,
                           int64_t w, const int64_t* pos, int64_t count, const float* measured, const int64_t* order,
                           int64_t B, float loss_scale, float* loss_per_position, float* amplitude, void* grad_sample,
-                          void* grad_probe, void* workspace, size_t ws_bytes, unsigned flags, hipStream_t s) {
+                          void* grad_probe, void* workspace, size_t ws_bytes, unsigned flags, hipStream_t s,
+                          const AdamArgs* adam_obj, const AdamArgs* adam_probe) {
   ptx_status st = check_shapes(fn, H, W, h, w, B);
   if (st != PTX_OK) return st;
   if (!grad_sample || !grad_probe) {
     set_error("%s: null gradient output", fn);
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  const AdamArgs ao = adam_obj ? *adam_obj : AdamArgs{};
+  const AdamArgs ap = adam_probe ? *adam_probe : AdamArgs{};
+  if ((ao.param || ap.param) && (flags & PTX_FLAG_ACCUMULATE)) {
+    set_error("%s: the fused Adam update takes the whole gradient (no PTX_FLAG_ACCUMULATE)", fn);
     return PTX_ERR_INVALID_ARGUMENT;
   }
   if (B == 0) {
@@ -591,7 +602,10 @@ ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t
       fill_zero2((float2*)grad_sample, H * W, s);
       fill_zero2((float2*)grad_probe, h * w, s);
     }
-    return check_launch("zero fill", s);
+    if ((st = check_launch("zero fill", s)) != PTX_OK) return st;
+    if (ao.param && (st = launch_adam(ao, (const float2*)grad_sample, H * W, s)) != PTX_OK) return st;
+    if (ap.param) return launch_adam(ap, (const float2*)grad_probe, h * w, s);
+    return PTX_OK;
   }
   if (!sample || !probe || !pos || !measured || !loss_per_position) {
     set_error("%s: null pointer argument", fn);
@@ -621,13 +635,13 @@ ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t
     switch (h) {
       case 32:
         return frame_step_launch<32>(ws, sp, smp, H, W, prb, bpos, measured, order, count, B, loss_scale,
-                                     loss_per_position, amplitude, gsp, gpp, flags, s);
+                                     loss_per_position, amplitude, gsp, gpp, flags, s, ao, ap);
       case 64:
         return frame_step_launch<64>(ws, sp, smp, H, W, prb, bpos, measured, order, count, B, loss_scale,
-                                     loss_per_position, amplitude, gsp, gpp, flags, s);
+                                     loss_per_position, amplitude, gsp, gpp, flags, s, ao, ap);
       default:
         return frame_step_launch<128>(ws, sp, smp, H, W, prb, bpos, measured, order, count, B, loss_scale,
-                                      loss_per_position, amplitude, gsp, gpp, flags, s);
+                                      loss_per_position, amplitude, gsp, gpp, flags, s, ao, ap);
     }
   }
   ScanGatherRows<true> r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, bpos, ws.tb};
@@ -638,7 +652,7 @@ ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t
   if (st != PTX_OK) return st;
   const LossRows loss{ws.loss_part, loss_parts, 1.0f / (float)(h * w), loss_per_position, B};
   return scan_scatter<true>(ws, sp, (const float2*)sample, H, W, (const float2*)probe, h, w, bpos, B,
-                            (float2*)grad_sample, (float2*)grad_probe, flags, 1.0f, s, loss);
+                            (float2*)grad_sample, (float2*)grad_probe, flags, 1.0f, s, loss, ao, ap);
 }
 
 }  // namespace ptx
@@ -674,6 +688,39 @@ ptx_status ptx_base_step_indexed(const void* sample, int64_t H, int64_t W, const
   return base_step_impl("ptx_base_step_indexed", sample, H, W, probe, h, w, positions_all, count, measured_all,
                         order, B, loss_scale, loss_per_position, amplitude, grad_sample, grad_probe, workspace,
                         ws_bytes, flags, (hipStream_t)stream);
+}
+
+ptx_status ptx_base_train_step(void* sample, int64_t H, int64_t W, void* probe, int64_t h, int64_t w,
+                               const int64_t* positions_all, const float* measured_all, int64_t count,
+                               const int64_t* order, int64_t B, float loss_scale, float* loss_per_position,
+                               float* amplitude, void* grad_sample, void* grad_probe, const ptx_adam* adam_sample,
+                               const ptx_adam* adam_probe, void* workspace, size_t ws_bytes, unsigned flags,
+                               void* stream) {
+  TuneScope tune_scope;
+  const char* fn = "ptx_base_train_step";
+  AdamArgs upd[2]{};
+  const ptx_adam* cfg[2] = {adam_sample, adam_probe};
+  void* params[2] = {sample, probe};
+  for (int k = 0; k < 2; ++k) {
+    const ptx_adam* a = cfg[k];
+    if (!a) continue;
+    if (!a->exp_avg || !a->exp_avg_sq || !a->hyper || !params[k]) {
+      set_error("%s: null Adam state", fn);
+      return PTX_ERR_INVALID_ARGUMENT;
+    }
+    if (!(a->beta1 >= 0.0f && a->beta1 < 1.0f && a->beta2 >= 0.0f && a->beta2 < 1.0f)) {
+      set_error("%s: betas must lie in [0, 1)", fn);
+      return PTX_ERR_INVALID_ARGUMENT;
+    }
+    upd[k] = AdamArgs{(float2*)params[k], a->exp_avg, a->exp_avg_sq, a->hyper, a->beta1, a->beta2};
+  }
+  if (order && B > 0 && (count <= 0 || count > INT32_MAX)) {
+    set_error("%s: table of %lld views (need 1 .. 2^31 - 1)", fn, (long long)count);
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  return base_step_impl(fn, sample, H, W, probe, h, w, positions_all, order ? count : B, measured_all, order, B,
+                        loss_scale, loss_per_position, amplitude, grad_sample, grad_probe, workspace, ws_bytes, flags,
+                        (hipStream_t)stream, &upd[0], &upd[1]);
 }
 
 }  // extern "C"

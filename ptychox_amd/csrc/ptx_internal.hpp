@@ -79,12 +79,19 @@ struct TuneScope {
   TuneScope& operator=(const TuneScope&) = delete;
 };
 
+struct AdamArgs;
+// The stand-alone Adam update of `adam` with gradient `grad` of n complex
+// elements (defined in ptx_optim.hip).
+ptx_status launch_adam(const AdamArgs& adam, const float2* grad, int64_t n, hipStream_t stream);
+
 // Body of the conventional fused step (ptx_base.hip), shared by its entry
-// points: ws_bytes of caller scratch at workspace (see ptx_base_step).
+// points: ws_bytes of caller scratch at workspace (see ptx_base_step);
+// adam_obj / adam_probe (may be null) fuse those Adam updates.
 ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t W, const void* probe, int64_t h,
                           int64_t w, const int64_t* pos, int64_t count, const float* measured, const int64_t* order,
                           int64_t B, float loss_scale, float* loss_per_position, float* amplitude, void* grad_sample,
-                          void* grad_probe, void* workspace, size_t ws_bytes, unsigned flags, hipStream_t s);
+                          void* grad_probe, void* workspace, size_t ws_bytes, unsigned flags, hipStream_t s,
+                          const AdamArgs* adam_obj = nullptr, const AdamArgs* adam_probe = nullptr);
 
 // Call F<N, E>::run(args...) for the runtime power-of-two n and the tuned E.
 template <template <int, int> class F, typename... Args>

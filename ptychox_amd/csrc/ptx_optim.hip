@@ -25,16 +25,11 @@ static __global__ void __launch_bounds__(256)
   const float step_size = hyper[0];
   const float bc2_sqrt = hyper[1];
   const float eps = hyper[2];
-  const float w1 = 1.0f - beta1;
-  const float w2 = 1.0f - beta2;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-    const float gi = g[i];
-    const float mi = m[i] + w1 * (gi - m[i]);  // lerp with weight < 0.5
-    const float vi = v[i] * beta2 + w2 * (gi * gi);
+    float mi = m[i], vi = v[i];
+    float pi = adam_component(p[i], g[i], mi, vi, beta1, beta2, step_size, bc2_sqrt, eps);
     m[i] = mi;
     v[i] = vi;
-    const float denom = sqrtf(vi) / bc2_sqrt + eps;
-    float pi = p[i] + (-step_size) * (mi / denom);
     if (PROJECT) pi = (i & 1) ? 0.0f : (pi < 0.0f ? 0.0f : pi);
     p[i] = pi;
   }
@@ -55,6 +50,16 @@ static __global__ void __launch_bounds__(256)
 }
 
 }  // namespace
+
+ptx_status launch_adam(const AdamArgs& adam, const float2* grad, int64_t n, hipStream_t s) {
+  if (n <= 0) return PTX_OK;
+  int64_t blocks = (2 * n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  k_adam<false><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(reinterpret_cast<float*>(adam.param),
+                                                             reinterpret_cast<const float*>(grad), adam.m, adam.v,
+                                                             2 * n, adam.beta1, adam.beta2, adam.hyper);
+  return check_launch("adam", s);
+}
 
 ptx_status launch_gather_positions(const int64_t* order, const int64_t* positions_all, int64_t count, int64_t batch,
                                    int64_t* positions_out, int* bad, hipStream_t s) {

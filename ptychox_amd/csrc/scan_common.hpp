@@ -305,10 +305,14 @@ ptx_status check_shapes(const char* fn, int64_t H, int64_t W, int64_t h, int64_t
 // gather.  The tile lists (they need only the positions) and the step's
 // per-view loss sums (`loss`, batch 0 for none) ride along with the scatter
 // launch, the probe-partial sum with the gather launch.
+// With adam_obj / adam_probe set (param non-null) the gradients feed those
+// Adam updates: fused into the gather / probe-sum epilogues, or run as the
+// stand-alone update after the atomic scatter (gs, gp are then scratch).
 template <bool SIGN>
 ptx_status scan_scatter(const ScanWorkspace& ws, const ScatterPlan& sp, const float2* sample, int64_t H, int64_t W,
                         const float2* probe, int64_t h, int64_t w, const int64_t* pos, int64_t B, float2* gs,
-                        float2* gp, unsigned flags, float scale, hipStream_t s, const LossRows& loss = LossRows{}) {
+                        float2* gp, unsigned flags, float scale, hipStream_t s, const LossRows& loss = LossRows{},
+                        const AdamArgs& adam_obj = AdamArgs{}, const AdamArgs& adam_probe = AdamArgs{}) {
   const bool det = use_segments(flags);
   const bool acc = flags & PTX_FLAG_ACCUMULATE;
   ptx_status st = PTX_OK;
@@ -323,14 +327,17 @@ ptx_status scan_scatter(const ScanWorkspace& ws, const ScatterPlan& sp, const fl
                                   det ? ws.tb : nullptr, ws.gp_part, scale, with_scatter, s,
                                   std::integral_constant<bool, SIGN>{});
   if (st != PTX_OK) return st;
-  const SumParts2 probe_sum{ws.gp_part, sp.nchunks, h * w, gp, acc ? 1 : 0, 0};
+  SumParts2 probe_sum{ws.gp_part, sp.nchunks, h * w, gp, acc ? 1 : 0, 0};
+  probe_sum.adam = adam_probe;
   if (!det) {
-    sum_parts2(ws.gp_part, sp.nchunks, h * w, gp, acc ? 1 : 0, s);
-    return check_launch("probe partial sum", s);
+    if (probe_sum.blocks() > 0) k_sum_parts2<<<dim3((unsigned)probe_sum.blocks()), dim3(256), 0, s>>>(probe_sum);
+    if ((st = check_launch("probe partial sum", s)) != PTX_OK) return st;
+    if (adam_obj.param) return launch_adam(adam_obj, gs, H * W, s);  // after the atomic scatter
+    return PTX_OK;
   }
   SideWork with_gather{};
   with_gather.sums = probe_sum;
-  return run_tile_gather(ws.gather, ws.tb, h, w, pos, B, 0, 0, gs, H, W, acc ? 1 : 0, s, with_gather);
+  return run_tile_gather(ws.gather, ws.tb, h, w, pos, B, 0, 0, gs, H, W, acc ? 1 : 0, s, with_gather, adam_obj);
 }
 
 }  // namespace

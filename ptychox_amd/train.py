@@ -58,6 +58,13 @@ class AdamState:
         self.beta1, self.beta2 = float(beta1), float(beta2)
         self.flags = _lib.PTX_ADAM_PROJECT_REAL_NONNEG if real_nonneg else 0
 
+    def native(self, hyper: torch.Tensor) -> "_lib.PtxAdam":
+        """This state as the ``ptx_adam`` a fused training step takes
+        (no projection: the fused steps update unconstrained unknowns)."""
+        if self.flags:
+            raise ValueError("the fused Adam update has no projection")
+        return _lib.PtxAdam(self.m.data_ptr(), self.v.data_ptr(), hyper.data_ptr(), self.beta1, self.beta2)
+
     def step(self, grad: torch.Tensor, hyper: torch.Tensor) -> None:
         """One update; ``hyper`` is a device float[3] (see :func:`adam_hyper`)."""
         _lib.call(
@@ -189,12 +196,16 @@ class ConventionalReconstruction(_EpochLoop):
     (``ptx_base_step_indexed``), so no mini-batch is gathered.
 
     ``graph=True`` captures one epoch the first time it runs and replays it.
+    ``fused_adam=True`` (the default) runs each mini-batch as one
+    ``ptx_base_train_step``: the Adam updates of object and probe are applied
+    in the epilogues that finish the gradients, with the same results as the
+    step followed by two ``ptx_adam_step`` calls (``fused_adam=False``).
     """
 
     def __init__(self, sample: torch.Tensor, probe: torch.Tensor, positions: torch.Tensor, measured: torch.Tensor,
                  batch_size: int, lr: float, probe_lr: float | None = None, min_lr: float = 0.0, epochs: int = 1,
                  generator: torch.Generator | None = None, graph: bool = True, scatter: str | None = None,
-                 betas=(0.9, 0.999), eps: float = 1e-16):
+                 betas=(0.9, 0.999), eps: float = 1e-16, fused_adam: bool = True):
         dev = sample.device
         positions = positions.to(device=dev, dtype=torch.int64).contiguous()
         super().__init__(sample, probe, positions.shape[0], batch_size, lr, probe_lr, min_lr, epochs, generator,
@@ -207,6 +218,7 @@ class ConventionalReconstruction(_EpochLoop):
         if self.measured.shape != (self.P, self.h, self.w):
             raise ValueError("measured must have shape [positions, probe_h, probe_w]")
         self.flags = scatter_flags(scatter)
+        self.fused_adam = bool(fused_adam)
         nbytes = _lib.query("ptx_base_step_workspace", self.H, self.W, self.h, self.w, self.batch, self.flags)
         self.ws = _lib.workspace(nbytes, dev)
 
@@ -217,6 +229,27 @@ class ConventionalReconstruction(_EpochLoop):
     @property
     def probe(self) -> torch.Tensor:
         return self.aux
+
+    def _launch_epoch(self) -> None:
+        if not self.fused_adam:
+            super()._launch_epoch()
+            return
+        import ctypes
+
+        for k in range(self.nb):
+            start = k * self.batch
+            B = min(self.batch, self.P - start)
+            ao = self.opt_o.native(self.hyper[0, k])
+            ap = self.opt_a.native(self.hyper[1, k]) if self.train_aux else None
+            _lib.call(
+                "ptx_base_train_step",
+                _lib.ptr(self.obj), self.H, self.W, _lib.ptr(self.aux), self.h, self.w,
+                _lib.ptr(self.positions), _lib.ptr(self.measured), self.P,
+                ctypes_offset(self.order, start), B, 1.0 / (B * self.h * self.w),
+                ctypes_offset(self.loss, start), None, _lib.ptr(self.g_obj), _lib.ptr(self.g_aux),
+                ctypes.c_void_p(ctypes.addressof(ao)), None if ap is None else ctypes.c_void_p(ctypes.addressof(ap)),
+                _lib.ptr(self.ws), self.ws.numel(), self.flags, _lib.stream_of(self.device),
+            )
 
     def _batch(self, k: int, start: int, B: int) -> None:
         _lib.call(

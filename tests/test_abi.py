@@ -97,6 +97,26 @@ def test_adam_argument_checks(lib):
     lib.call("ptx_adam_step", *args, 0, 0.9, 0.999, fake, 0, None)  # n = 0: nothing to do
 
 
+def test_fused_adam_step_argument_checks(lib):
+    # rejected on the host before anything is launched
+    fake = ctypes.c_void_p(256)
+
+    def call(adam_s, adam_p, flags=0, count=8):
+        refs = [None if a is None else ctypes.c_void_p(ctypes.addressof(a)) for a in (adam_s, adam_p)]
+        lib.call("ptx_base_train_step", fake, 256, 256, fake, 64, 64, fake, fake, count, fake, 4, 1.0, fake, None,
+                 fake, fake, *refs, fake, 1 << 30, flags, None)
+
+    good = lib.PtxAdam(256, 256, 256, 0.9, 0.999)
+    with pytest.raises(ValueError, match="betas"):
+        call(lib.PtxAdam(256, 256, 256, 1.0, 0.999), None)
+    with pytest.raises(ValueError, match="null Adam state"):
+        call(good, lib.PtxAdam(256, 0, 256, 0.9, 0.999))
+    with pytest.raises(ValueError, match="ACCUMULATE"):
+        call(good, good, flags=lib.PTX_FLAG_ACCUMULATE)
+    with pytest.raises(ValueError, match="table of"):
+        call(good, good, count=0)
+
+
 def test_view_table_checks(lib):
     # an empty (or 2^31-entry) table for a non-empty batch is rejected on the
     # host, before anything is launched
