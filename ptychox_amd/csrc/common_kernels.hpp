@@ -352,8 +352,10 @@ static __global__ void __launch_bounds__(1024)
 // SEG: one work item per segment (k_gather_plan's enumeration); otherwise one
 // work item per tile over its whole list, staged through LDS kListChunk views
 // at a time.
+// ADAM: the tile's finished gradient feeds `adam` (a separate instantiation,
+// so the plain gather carries none of the update's code).
 constexpr int kListChunk = 64;
-template <bool SEG>
+template <bool SEG, bool ADAM>
 static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __restrict__ contrib, int h, int w,
                                                             const int64_t* __restrict__ pos, int64_t batch,
                                                             int64_t off_y, int64_t off_x,
@@ -457,7 +459,7 @@ static __global__ void __launch_bounds__(256) k_tile_gather(const float2* __rest
   for (int j = 0; j < RPT; ++j) {
     const int64_t Y = Y0 + j * RSTEP;
     if (Y < H) {
-      if (adam.param && final_here) {
+      if (ADAM && final_here) {
         adam.apply(Y * W + X, acc[j]);
       } else {
         float2* d = grad + Y * W + X;
@@ -587,17 +589,29 @@ inline ptx_status run_tile_gather(const GatherScratch& g, const float2* contrib,
   const TileGrid tg = tile_grid(h, w, B, H, W);
   ptx_status st;
   if (!gather_segmented(h, w, B, H, W)) {
-    k_tile_gather<false><<<dim3((unsigned)(tg.count() + side.blocks())), dim3(256), 0, s>>>(
-        contrib, (int)h, (int)w, pos, B, off_y, off_x, g.lists, g.counts, tg.tiles_x, nullptr, nullptr, nullptr,
-        nullptr, nullptr, grad, H, W, accumulate, side, tg.count(), adam);
+    const dim3 grid((unsigned)(tg.count() + side.blocks()));
+    if (adam.param)
+      k_tile_gather<false, true><<<grid, dim3(256), 0, s>>>(
+          contrib, (int)h, (int)w, pos, B, off_y, off_x, g.lists, g.counts, tg.tiles_x, nullptr, nullptr, nullptr,
+          nullptr, nullptr, grad, H, W, accumulate, side, tg.count(), adam);
+    else
+      k_tile_gather<false, false><<<grid, dim3(256), 0, s>>>(
+          contrib, (int)h, (int)w, pos, B, off_y, off_x, g.lists, g.counts, tg.tiles_x, nullptr, nullptr, nullptr,
+          nullptr, nullptr, grad, H, W, accumulate, side, tg.count(), adam);
     return check_launch("tile gather", s);
   }
   const GatherBounds gb = gather_bounds(tg.count(), h, w, B);
   k_gather_plan<<<dim3(1), dim3(1024), 0, s>>>(g.counts, tg.count(), g.seg_off, g.slot_off, g.item_tile, g.n_items);
   if ((st = check_launch("gather plan", s)) != PTX_OK) return st;
-  k_tile_gather<true><<<dim3((unsigned)(gb.items + side.blocks())), dim3(256), 0, s>>>(
-      contrib, (int)h, (int)w, pos, B, off_y, off_x, g.lists, g.counts, tg.tiles_x, g.seg_off, g.slot_off,
-      g.item_tile, g.n_items, g.partial, grad, H, W, accumulate, side, gb.items, adam);
+  const dim3 grid((unsigned)(gb.items + side.blocks()));
+  if (adam.param)
+    k_tile_gather<true, true><<<grid, dim3(256), 0, s>>>(
+        contrib, (int)h, (int)w, pos, B, off_y, off_x, g.lists, g.counts, tg.tiles_x, g.seg_off, g.slot_off,
+        g.item_tile, g.n_items, g.partial, grad, H, W, accumulate, side, gb.items, adam);
+  else
+    k_tile_gather<true, false><<<grid, dim3(256), 0, s>>>(
+        contrib, (int)h, (int)w, pos, B, off_y, off_x, g.lists, g.counts, tg.tiles_x, g.seg_off, g.slot_off,
+        g.item_tile, g.n_items, g.partial, grad, H, W, accumulate, side, gb.items, adam);
   if ((st = check_launch("tile gather", s)) != PTX_OK) return st;
   if (gb.slots > 0) {
     k_gather_combine<<<dim3((unsigned)(tg.count() * (kSegTilePx / 256))), dim3(256), 0, s>>>(g.counts, g.slot_off,

@@ -134,7 +134,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     int64_t mframe = ti.frame;
     if (order) {
       const uint64_t o = (uint64_t)order[ti.frame];
-      // out of range (flagged by the position gather): frame 0
+      // out of range: frame 0 (the positions of such a view are entry 0's too)
       mframe = ((o >> 32) == 0 && (uint32_t)o < (uint32_t)count) ? (int64_t)o : 0;
     }
     const auto rt = buffer_rsrc(tb + ti.frame * fsz, live ? fbytes : 0u);
@@ -621,13 +621,15 @@ ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t
     return PTX_ERR_WORKSPACE;
   }
   timing_begin(s);
-  const int64_t* bpos = pos;
-  if (order) {
-    fill_zero2(reinterpret_cast<float2*>(ws.bad_index), 1, s);
-    if ((st = launch_gather_positions(order, pos, count, B, ws.pos_batch, ws.bad_index, s)) != PTX_OK) return st;
-    bpos = ws.pos_batch;
-  }
-  if (frame_step_eligible(H, W, h, w, flags)) {
+  const bool frame = frame_step_eligible(H, W, h, w, flags);
+  // an indexed batch's positions are gathered by the gather-rows kernel itself
+  // when it is the view-looping one, else by a kernel of their own first
+  const bool order_in_gather = order && !frame && gather_loop_ok(H, W, h, w);
+  const int64_t* bpos = order ? ws.pos_batch : pos;
+  if (order && !order_in_gather &&
+      (st = launch_gather_positions(order, pos, count, B, ws.pos_batch, nullptr, s)) != PTX_OK)
+    return st;
+  if (frame) {
     auto* smp = (const float2*)sample;
     auto* prb = (const float2*)probe;
     auto* gsp = (float2*)grad_sample;
@@ -645,6 +647,12 @@ ptx_status base_step_impl(const char* fn, const void* sample, int64_t H, int64_t
     }
   }
   ScanGatherRows<true> r1{(const float2*)sample, H, W, (const float2*)probe, (int)h, bpos, ws.tb};
+  if (order_in_gather) {
+    r1.pos = pos;
+    r1.order = order;
+    r1.count = count;
+    r1.pos_out = ws.pos_batch;
+  }
   st = dispatch_ne<GatherRowsRun>(PASS_ROW, w, r1, B * h, s);
   if (st != PTX_OK) return st;
   st = dispatch_ne<StepColsRun>(PASS_COL, h, ws.tb, measured, order, count, amplitude, ws.loss_part, B, w,

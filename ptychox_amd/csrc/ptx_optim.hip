@@ -42,7 +42,7 @@ static __global__ void __launch_bounds__(256)
   if (b >= B) return;
   int64_t k = order[b];
   if (k < 0 || k >= count) {  // never read outside the position table
-    atomicOr(bad, 1);
+    if (bad) atomicOr(bad, 1);
     k = 0;
   }
   out[2 * b] = pos_all[2 * k];

@@ -23,6 +23,12 @@ struct ScanGatherRows {
   int h;
   const int64_t* pos;
   float2* out;  // [B, h, w]
+  // Indexed batches (the view-looping kernel only): view b is entry order[b]
+  // of pos[count] (an entry outside [0, count) reads entry 0), and work-group
+  // 0 writes the batch's positions to pos_out for the passes that follow.
+  const int64_t* order = nullptr;
+  int64_t count = 0;
+  int64_t* pos_out = nullptr;
   template <int N, int E>
   __device__ __forceinline__ void load(int64_t row, bool valid, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
@@ -60,12 +66,19 @@ struct ScanGatherRows {
 // out-of-range offset, and such rows are stored to an out-of-range offset
 // (dropped), so the compiler's wait counts keep the prefetch in flight.  The
 // exchanges alternate two LDS regions.  Same arithmetic per element as
-// ScanGatherRows.  Needs the object below 4 GiB (32-bit buffer offsets).
+// ScanGatherRows.  Needs the object below 2 GiB (32-bit buffer offsets).
+// order != null: view b is position order[b] of pos[count] (entry 0 for an
+// order entry outside [0, count)), and work-group 0 also writes the batch's
+// positions to pos_out, so no separate gather of the positions runs before.
+inline bool gather_loop_ok(int64_t H, int64_t W, int64_t h, int64_t n) {
+  return H * W * 8 < (int64_t(1) << 31) && h * n * 8 < (int64_t(1) << 31);
+}
 template <int N, int E, bool SIGN>
 __global__ void __launch_bounds__(256)
     k_scan_gather_loop(const float2* __restrict__ sample, int64_t H, int64_t W, const float2* __restrict__ probe,
                        int h, const int64_t* __restrict__ pos, int64_t B, int chunk, int ngroups,
-                       float2* __restrict__ out) {
+                       float2* __restrict__ out, const int64_t* __restrict__ order, int64_t count,
+                       int64_t* __restrict__ pos_out) {
   using Cf = RowCfg<N, E>;
   constexpr int T = Cf::T, S = Cf::S;
   constexpr bool ODD = (FftShape<N, E>::STAGES - 1) % 2 == 1;
@@ -92,10 +105,22 @@ __global__ void __launch_bounds__(256)
     pr[q] = (SIGN && ((r + i) & 1)) ? c_neg(p) : p;
   }
   const uint32_t out_lane = rv ? (uint32_t)(((int64_t)r * N + t) * 8) : kOut;
+  auto entry = [&](int64_t b) -> int64_t {
+    if (!order) return b;
+    const int64_t k = order[b];
+    return (uint64_t)k < (uint64_t)count ? k : 0;
+  };
+  if (pos_out && blockIdx.x == 0) {
+    for (int64_t b = threadIdx.x; b < B; b += 256) {
+      const int64_t k = entry(b);
+      pos_out[2 * b] = pos[2 * k];
+      pos_out[2 * b + 1] = pos[2 * k + 1];
+    }
+  }
   auto fetch = [&](unsigned b, float2 (&dv)[E]) {
     const bool live = b < b1;
-    const unsigned bb = live ? b : b0;
-    const int64_t x0 = pos[2 * bb], y0 = pos[2 * bb + 1];
+    const int64_t k = entry(live ? b : b0);
+    const int64_t x0 = pos[2 * k], y0 = pos[2 * k + 1];
     const bool ok = live && window_ok(x0, y0, h, N, H, W);
     const auto rs = buffer_rsrc(sample, ok ? obj_bytes : 0u);
     const uint32_t voff = (ok && rv) ? (uint32_t)(((y0 + r) * W + x0 + t) * 8) : kOut;
@@ -203,7 +228,7 @@ struct GatherRowsRun {
   static ptx_status run(const ScanGatherRows<SIGN>& op, int64_t nrows, hipStream_t s) {
     using Cf = RowCfg<N, E>;
     const int64_t B = op.h > 0 ? nrows / op.h : 0;
-    if (B > 0 && op.H * op.W * 8 < (int64_t(1) << 31) && (int64_t)op.h * N * 8 < (int64_t(1) << 31)) {
+    if (B > 0 && gather_loop_ok(op.H, op.W, op.h, N)) {
       // probe rows resident: ~1024 work-groups (tunable), each walking a chunk of views
       const int ngroups = (int)((op.h + Cf::S - 1) / Cf::S);
       int64_t nchunks = (tuned_groups(PASS_ROW, N, 1024) + ngroups - 1) / ngroups;
@@ -211,8 +236,13 @@ struct GatherRowsRun {
       const int64_t chunk = (B + nchunks - 1) / nchunks;
       nchunks = (B + chunk - 1) / chunk;
       k_scan_gather_loop<N, E, SIGN><<<dim3((unsigned)(ngroups * nchunks)), dim3(256), 0, s>>>(
-          op.sample, op.H, op.W, op.probe, op.h, op.pos, B, (int)chunk, ngroups, op.out);
+          op.sample, op.H, op.W, op.probe, op.h, op.pos, B, (int)chunk, ngroups, op.out, op.order, op.count,
+          op.pos_out);
     } else {
+      if (op.order) {
+        set_error("scan gather rows: indexed views need the view-looping kernel");
+        return PTX_ERR_INVALID_ARGUMENT;
+      }
       launch_rows<N, E, -1>(op, nrows, s);
     }
     return check_launch(SIGN ? "base gather rows" : "scan gather rows", s);
@@ -263,7 +293,6 @@ struct ScanWorkspace {
   float* loss_part;  // [B, loss partials per frame]   (step only)
   GatherScratch gather;  // deterministic only
   int64_t* pos_batch;  // [B, 2] gathered positions (indexed step only)
-  int* bad_index;      // set when an order entry is out of range (indexed step only)
   size_t bytes;
 };
 
@@ -275,10 +304,7 @@ ScanWorkspace carve_scan(void* base, int64_t H, int64_t W, int64_t h, int64_t w,
   ws.gp_part = cv.take<float2>((size_t)(sp.nchunks * h * w));
   ws.loss_part = step ? cv.take<float>((size_t)(B * loss_parts)) : nullptr;
   if (use_segments(flags)) ws.gather = carve_gather(cv, H, W, h, w, B);
-  if (indexed) {
-    ws.pos_batch = cv.take<int64_t>((size_t)(2 * B));
-    ws.bad_index = reinterpret_cast<int*>(cv.take<float2>(1));  // zeroed with the float2 fill kernel
-  }
+  if (indexed) ws.pos_batch = cv.take<int64_t>((size_t)(2 * B));
   ws.bytes = cv.used;
   return ws;
 }

@@ -174,3 +174,43 @@ def test_nearfield_skips_view(gpu, one_probe_slice, mode, where):
     (gs1, gp1, l1), (gs0, gp0, l0) = res
     _compare(mode, (gs1, gp1), (gs0, gp0))
     assert torch.equal(l1[keep], l0)
+
+
+@pytest.mark.parametrize("side", [64, 32])  # three-pass path (positions gathered in the gather rows), whole frame
+@pytest.mark.parametrize("mode", ["segments", "atomic"])
+def test_base_step_indexed_order_outside_table(gpu, mode, side):
+    # an order entry outside [0, count) reads entry 0 of both tables (include/ptx.h):
+    # bitwise the indexed step whose order holds 0 there
+    from ptychox_amd import _lib
+    from ptychox_amd._utils import scatter_flags
+
+    rng = np.random.default_rng(21 + side)
+    Hs = 4 * side
+    s = (smooth_object(rng, Hs) + 0.05 * crandn(rng, Hs, Hs)).astype(np.complex64)
+    p = disc_probe(side, side / 4)
+    count, n = 30, 12
+    pos_all = rng.integers(0, Hs - side + 1, (count, 2)).astype(np.int64)
+    y_all = rng.uniform(0.0, 2.0, (count, side, side)).astype(np.float32)
+    order = rng.permutation(count)[:n].astype(np.int64)
+    bad = order.copy()
+    bad[3], bad[7] = -1, count + 5
+    good = order.copy()
+    good[3], good[7] = 0, 0
+    sd, pd, pa, ya = to_dev(s, gpu), to_dev(p, gpu), to_dev(pos_all, gpu), to_dev(y_all, gpu)
+    flags = scatter_flags(mode)
+    ws = _lib.workspace(_lib.query("ptx_base_step_workspace", Hs, Hs, side, side, n, flags), gpu)
+    res = []
+    for od in (bad, good):
+        gs, gp = torch.empty_like(sd), torch.empty_like(pd)
+        loss = torch.empty(n, dtype=torch.float32, device=gpu)
+        _lib.call("ptx_base_step_indexed", _lib.ptr(sd), Hs, Hs, _lib.ptr(pd), side, side, _lib.ptr(pa),
+                  _lib.ptr(ya), count, _lib.ptr(to_dev(od, gpu)), n, 1.0 / (n * side * side), _lib.ptr(loss), None,
+                  _lib.ptr(gs), _lib.ptr(gp), _lib.ptr(ws), ws.numel(), flags, _lib.stream_of(gpu))
+        torch.cuda.synchronize()
+        res.append((gs, gp, loss))
+    (gs1, gp1, l1), (gs0, gp0, l0) = res
+    assert torch.equal(l1, l0)
+    if mode == "segments":
+        assert torch.equal(gs1, gs0) and torch.equal(gp1, gp0)
+    else:
+        assert nrel(gs1, gs0) < 1e-6 and nrel(gp1, gp0) < 1e-6
