@@ -107,8 +107,9 @@ const char* ptx_timing_label(int k);
  * length.  kind: 0 row passes, 1 row passes that loop over a chunk of views
  * (object / probe accumulation), 2 column passes, 3 column passes that
  * loop over views, 4 row passes that loop over views keeping several frames'
- * values per thread (coded mask gradient).  elems: 8 or 16, 0 restores the
- * built-in default.
+ * values per thread (coded mask gradient), 5 row passes that evaluate the
+ * amplitude loss between two transforms (coded step).  elems: 8 or 16, 0
+ * restores the built-in default.
  * Process-wide; set it before launching work (the autotuner in
  * ptychox_amd/autotune.py measures and fills it). */
 ptx_status ptx_tune_set(int kind, int64_t n, int elems);
@@ -119,7 +120,8 @@ int ptx_tune_get(int kind, int64_t n);
  * 1024), 1 the sorted-segment scatters (1024), 2 the transfer-function
  * column passes (one work-group per CU) and the coded spectrum column pass
  * (256), 3 the coded spectrum-gradient column pass (256), 4 the coded
- * mask-gradient row pass (2048).  The chunk of views per work-group follows
+ * mask-gradient row pass (2048); kind 5 has no view chunks (the setting is
+ * kept but unused).  The chunk of views per work-group follows
  * from it; for kinds 1, 3 and 4 so does the number of partial slices, hence
  * the workspace size (query the *_workspace functions after changing it).
  * groups: 1..65536 (1: every work-group walks all views of its rows or

@@ -25,13 +25,13 @@ import torch
 
 from . import _lib
 
-PASS_ROW, PASS_ROW_LOOP, PASS_COL, PASS_COL_LOOP, PASS_VIEW_ROWS = 0, 1, 2, 3, 4
+PASS_ROW, PASS_ROW_LOOP, PASS_COL, PASS_COL_LOOP, PASS_VIEW_ROWS, PASS_LOSS_ROWS = 0, 1, 2, 3, 4, 5
 KIND_NAMES = {PASS_ROW: "row", PASS_ROW_LOOP: "row_loop", PASS_COL: "col", PASS_COL_LOOP: "col_loop",
-              PASS_VIEW_ROWS: "view_rows"}
+              PASS_VIEW_ROWS: "view_rows", PASS_LOSS_ROWS: "loss_rows"}
 CANDIDATES = (8, 16)
 # work-group targets tried per pass kind (0: the built-in default)
 GROUP_CANDIDATES = {PASS_ROW: (0, 256, 512, 2048), PASS_ROW_LOOP: (0, 512, 2048), PASS_COL: (0, 128, 512),
-                    PASS_COL_LOOP: (0, 128, 512), PASS_VIEW_ROWS: (0, 1024, 4096)}
+                    PASS_COL_LOOP: (0, 128, 512), PASS_VIEW_ROWS: (0, 1024, 4096), PASS_LOSS_ROWS: (0,)}
 GROUPS_SUFFIX = "@groups"
 
 
@@ -148,7 +148,7 @@ def keys_for_fpm(object_shape, pupil_shape) -> list[tuple[int, int]]:
 
 def keys_for_coded(shape) -> list[tuple[int, int]]:
     H, W = shape
-    return [(PASS_ROW, W), (PASS_VIEW_ROWS, W), (PASS_COL, H), (PASS_COL_LOOP, H)]
+    return [(PASS_ROW, W), (PASS_LOSS_ROWS, W), (PASS_VIEW_ROWS, W), (PASS_COL, H), (PASS_COL_LOOP, H)]
 
 
 _env = os.environ.get("PTYCHOX_TUNING")

@@ -134,6 +134,39 @@ inline void launch_cols(const Op& op, int64_t frames, int64_t width, hipStream_t
   }
 }
 
+// ---------------------------------------------------------------- column-strip layout
+// A [rows, width] frame stored as strips of 16 adjacent columns (8 when the
+// width is 8), each strip row-major [rows][16]: element (y, x) at
+// ((x/16) rows + y) 16 + x%16.  A strip row is one 128-byte line, so the row
+// pass that writes the frame stores whole lines, and the column passes read
+// an 8- or 16-column tile of it from one contiguous block (the two 8-column
+// tiles of a strip run back to back on one XCD and share its lines in L2)
+// instead of pieces of lines 8 KB apart, which HBM serves at a fraction of the
+// streaming rate.
+__host__ __device__ constexpr uint32_t strip_lg(int64_t width) { return width >= 16 ? 4u : 3u; }
+__host__ __device__ __forceinline__ uint32_t strip_off(uint32_t y, uint32_t x, uint32_t rows, uint32_t lg) {
+  return (((x >> lg) * rows + y) << lg) + (x & ((1u << lg) - 1u));
+}
+
+// Row-pass store of v[q] = element (y, t + q T) of one frame (N columns, `rows`
+// rows) into its column-strip layout.  With T a multiple of the strip width
+// the q steps are whole strips, a scalar offset: one address register per lane.
+template <int N, int E>
+__device__ __forceinline__ void store_row_strip(float2* frame, uint32_t rows, uint32_t y, int t,
+                                                const float2 (&v)[E]) {
+  constexpr int T = N / E;
+  constexpr uint32_t lg = strip_lg(N);
+  const auto r = buffer_rsrc(frame, (uint32_t)N * rows * 8u);
+  if constexpr (T % (1 << lg) == 0) {
+    const uint32_t base = strip_off(y, (uint32_t)t, rows, lg) * 8u;
+#pragma unroll
+    for (int q = 0; q < E; ++q) buf_store_f2(v[q], r, base, (uint32_t)q * (uint32_t)T * rows * 8u);
+  } else {
+#pragma unroll
+    for (int q = 0; q < E; ++q) buf_store_f2(v[q], r, strip_off(y, (uint32_t)(t + q * T), rows, lg) * 8u, 0u);
+  }
+}
+
 // ---------------------------------------------------------------- transfer-function multiply
 // Applies F^-1_cols D F_cols to the columns of row-transformed frames: with the
 // rows already (inverse-)transformed, this completes  F^-1 (D . F x)  up to the
@@ -153,6 +186,8 @@ inline void launch_cols(const Op& op, int64_t frames, int64_t width, hipStream_t
 // regions with one barrier each; the inverse transform starts on the region
 // after the forward's last, so consecutive exchanges never share a region and
 // no other barrier is needed.
+// STRIP: the frames are read from `src` in the column-strip layout and the
+// results stored to `tb` in the plain layout (otherwise in place in `tb`).
 // REG_TW: the kernel holds its stage twiddles in LoopTwiddles<N, E, REG_TW>
 template <int N, int E, int C, bool REG_TW = true>
 struct PipeColsShape {
@@ -167,9 +202,9 @@ struct PipeColsShape {
                                                                                   : (int)((160 * 1024) / LDS_BYTES);
 };
 
-template <int N, int E, int C, bool CONJ>
+template <int N, int E, int C, bool CONJ, bool STRIP>
 __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
-    k_transfer_cols(float2* __restrict__ tb, const float2* __restrict__ transfer, int64_t width,
+    k_transfer_cols(float2* tb, const float2* src, const float2* __restrict__ transfer, int64_t width,
                     int64_t tiles_per_frame, int64_t frames, int64_t ngroups) {
   using Sh = PipeColsShape<N, E, C>;
   constexpr int T = Sh::T;
@@ -195,11 +230,20 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
 #pragma unroll
     for (int q = 0; q < E; ++q) tr[q] = buf_load_f2(rtr, lane * 8u, (q * rstep + col0) * 8u);
   }
+  // strip layout: row t of column col0 + c, then T rows (T strip rows) further per q
+  const uint32_t lg = strip_lg(width);
+  const uint32_t sbase = strip_off(0u, col0, (uint32_t)N, lg);
+  const uint32_t slane = strip_off((uint32_t)t, col0 + (uint32_t)c, (uint32_t)N, lg) - sbase;
   auto fetch = [&](unsigned f, float2 (&dv)[E]) {
     const bool live = f < nf;
-    const auto rt = buffer_rsrc(tb + (int64_t)(live ? f : 0u) * fsz, live ? fbytes : 0u);
+    const auto rt = buffer_rsrc((STRIP ? src : tb) + (int64_t)(live ? f : 0u) * fsz, live ? fbytes : 0u);
 #pragma unroll
-    for (int q = 0; q < E; ++q) dv[q] = buf_load_f2(rt, lane * 8u, (q * rstep + col0) * 8u);
+    for (int q = 0; q < E; ++q) {
+      if constexpr (STRIP)
+        dv[q] = buf_load_f2(rt, slane * 8u, (sbase + ((uint32_t)(q * T) << lg)) * 8u);
+      else
+        dv[q] = buf_load_f2(rt, lane * 8u, (q * rstep + col0) * 8u);
+    }
   };
   auto process = [&](unsigned f, float2 (&v)[E]) {
     // the exchange addresses depend only on (t, c); recompute them per frame
@@ -242,8 +286,8 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
 // work-groups, ngroups = the frame groups that fill the resident slots (or the
 // tuned work-group target of the column passes)
 template <int N, int E, int C>
-inline void launch_transfer_cols_c(float2* tb, const float2* transfer, int64_t frames, int64_t width, bool conj,
-                                   hipStream_t s) {
+inline void launch_transfer_cols_c(float2* tb, const float2* src, const float2* transfer, int64_t frames,
+                                   int64_t width, bool conj, hipStream_t s) {
   using Sh = PipeColsShape<N, E, C>;
   const int64_t tiles = width / C;
   // work-groups: the resident slots by default, or the tuned target of the column passes
@@ -251,23 +295,33 @@ inline void launch_transfer_cols_c(float2* tb, const float2* transfer, int64_t f
   if (ngroups > frames) ngroups = frames;
   if (ngroups < 1) ngroups = 1;
   const dim3 grid((unsigned)(tiles * ngroups));
-  if (conj)
-    k_transfer_cols<N, E, C, true><<<grid, dim3(Sh::THREADS), 0, s>>>(tb, transfer, width, tiles, frames, ngroups);
-  else
-    k_transfer_cols<N, E, C, false><<<grid, dim3(Sh::THREADS), 0, s>>>(tb, transfer, width, tiles, frames, ngroups);
+  const dim3 block(Sh::THREADS);
+  if (src) {
+    if (conj)
+      k_transfer_cols<N, E, C, true, true><<<grid, block, 0, s>>>(tb, src, transfer, width, tiles, frames, ngroups);
+    else
+      k_transfer_cols<N, E, C, false, true><<<grid, block, 0, s>>>(tb, src, transfer, width, tiles, frames, ngroups);
+  } else {
+    if (conj)
+      k_transfer_cols<N, E, C, true, false><<<grid, block, 0, s>>>(tb, tb, transfer, width, tiles, frames, ngroups);
+    else
+      k_transfer_cols<N, E, C, false, false><<<grid, block, 0, s>>>(tb, tb, transfer, width, tiles, frames, ngroups);
+  }
 }
 
+// src == null: in place in tb; otherwise from src in the column-strip layout
+// (width a multiple of 8) into tb
 template <int N, int E>
 inline void launch_transfer_cols(float2* tb, const float2* transfer, int64_t frames, int64_t width, bool conj,
-                                 hipStream_t s) {
+                                 hipStream_t s, const float2* src = nullptr) {
   using Cf = ColCfg<N, E>;
   if constexpr (Cf::CMAX > 8) {
     if (width < Cf::CMAX) {  // narrow frames: 8 columns per tile
-      launch_transfer_cols_c<N, E, 8>(tb, transfer, frames, width, conj, s);
+      launch_transfer_cols_c<N, E, 8>(tb, src, transfer, frames, width, conj, s);
       return;
     }
   }
-  launch_transfer_cols_c<N, E, Cf::CMAX>(tb, transfer, frames, width, conj, s);
+  launch_transfer_cols_c<N, E, Cf::CMAX>(tb, src, transfer, frames, width, conj, s);
 }
 
 // ---------------------------------------------------------------- plain ops

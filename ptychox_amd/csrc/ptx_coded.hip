@@ -20,7 +20,11 @@
 // row FFT) | Q4 columns over a chunk of views (column FFT, spectrum partials),
 // then fixed-order partial sums and the object-size inverse FFT.  Both shared
 // gradients accumulate in registers over the chunk: no atomics anywhere, so
-// the operator is deterministic.
+// the operator is deterministic.  The row passes that feed a column pass (P2,
+// Q1, Q3) write their frames in the column-strip layout (passes.hpp) into a
+// second frame buffer, so every column pass that reads frames (P3, Q2, Q4)
+// reads its tiles from contiguous strips; the column passes write the plain
+// layout, which the row passes read along their rows.
 #include "buffer_io.hpp"
 #include "common_kernels.hpp"
 #include "passes.hpp"
@@ -87,11 +91,11 @@ struct CodedSpecCols {
   }
 };
 
-// P2: row iFFT -> X = v/N (stored), * C, row FFT (in place)
+// P2: row iFFT -> X = v/N (stored), * C, row FFT -> out in the column-strip layout
 template <int N, int E>
 __global__ void __launch_bounds__(256)
-    k_coded_x_rows(float2* __restrict__ tb, float2* __restrict__ xfwd, const float2* __restrict__ mask, int64_t H,
-                   int64_t nrows, float inv_n) {
+    k_coded_x_rows(const float2* __restrict__ tb, float2* __restrict__ out, float2* __restrict__ xfwd,
+                   const float2* __restrict__ mask, int64_t H, int64_t nrows, float inv_n) {
   using Cf = RowCfg<N, E>;
   constexpr int T = Cf::T, S = Cf::S;
   __shared__ float2 lds[S * lds_len<N>()];
@@ -113,10 +117,7 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
-  if (valid) {
-#pragma unroll
-    for (int q = 0; q < E; ++q) tb[row * N + t + q * T] = v[q];
-  }
+  if (valid) store_row_strip<N, E>(out + (row - y) * N, (uint32_t)H, (uint32_t)y, t, v);
 }
 
 // P4 (forward): row iFFT -> Z = v/N stored; ds == 1: amplitude |Z|
@@ -179,7 +180,8 @@ static __global__ void __launch_bounds__(256)
 }
 
 // Q1: gZ = Z * sc_up (ds > 1, scale map from k_coded_pool) or
-// gZ = Z G/max(|Z|, eps) (ds == 1, straight from the upstream gradient); row FFT -> out
+// gZ = Z G/max(|Z|, eps) (ds == 1, straight from the upstream gradient); row FFT
+// -> out in the column-strip layout
 struct CodedGradZRows {
   const float2* z;
   const float* sc;  // ds > 1: [B, H/ds, W/ds] scale map;  ds == 1: G [B, H, W]
@@ -211,21 +213,23 @@ struct CodedGradZRows {
   __device__ __forceinline__ void store(int64_t row, bool valid, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
     if (!valid) return;
-#pragma unroll
-    for (int q = 0; q < E; ++q) out[row * N + t + q * T] = v[q];
+    const int64_t y = row % H;
+    float2* d = out + (row - y) * N;  // the frame, column-strip layout
+    store_row_strip<N, E>(d, (uint32_t)H, (uint32_t)y, t, v);
   }
 };
 
 // P4 + Q1 fused for the ds == 1 step: row iFFT -> Z, A = |Z|, loss partial per
-// row, gZ = Z gscale (A - Y)/max(A, eps), row FFT (in place)
+// row, gZ = Z gscale (A - Y)/max(A, eps), row FFT -> out in the column-strip layout
 template <int N, int E>
 __global__ void __launch_bounds__(256)
-    k_coded_step_rows(float2* __restrict__ tb, const float* __restrict__ measured, float* __restrict__ amp_out,
-                      float* __restrict__ loss_part, int64_t nrows, float gscale, float inv_n) {
+    k_coded_step_rows(const float2* __restrict__ tb, float2* __restrict__ out, const float* __restrict__ measured,
+                      float* __restrict__ amp_out, float* __restrict__ loss_part, int64_t H, int64_t nrows,
+                      float gscale, float inv_n) {
   using Cf = RowCfg<N, E>;
   constexpr int T = Cf::T, S = Cf::S;
   __shared__ float2 lds[S * lds_len<N>()];
-  __shared__ float red[256];
+  __shared__ float red[4];  // per-wave loss sums
   const int seq = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const int64_t row = (int64_t)blockIdx.x * S + seq;
@@ -246,26 +250,37 @@ __global__ void __launch_bounds__(256)
     if (valid && amp_out) amp_out[k] = a;
     v[q] = c_scale(e, gscale * d * ai.inv);
   }
-  red[threadIdx.x] = lsum;
-  __syncthreads();
-  if (valid && t == 0) {
-    float sum = 0.0f;
-    for (int k = 0; k < T; ++k) sum += red[seq * T + k];
-    loss_part[row] = sum;
+  // row sum: butterfly within each wave, then the row's waves in order (a row
+  // is T >= 64 threads, whole waves; shorter rows share a wave, T-lane groups)
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1)
+    if (off < T) lsum += __shfl_xor(lsum, off, 64);
+  if constexpr (T >= 64) {
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lsum;
+    __syncthreads();
+    if (valid && t == 0) {
+      float sum = 0.0f;
+      for (int k = 0; k < T / 64; ++k) sum += red[(seq * T >> 6) + k];
+      loss_part[row] = sum;
+    }
+  } else {
+    if (valid && t == 0) loss_part[row] = lsum;
+    __syncthreads();
   }
   fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
   if (valid) {
-#pragma unroll
-    for (int q = 0; q < E; ++q) tb[row * N + t + q * T] = v[q];
+    const int64_t y = row % H;
+    store_row_strip<N, E>(out + (row - y) * N, (uint32_t)H, (uint32_t)y, t, v);
   }
 }
 
 // Q3: per (row group, chunk of views): row iFFT -> gY = v/N; mask partial += gY conj(X_b);
-// gX = gY conj(C); row FFT (in place)
+// gX = gY conj(C); row FFT -> out in the column-strip layout
 template <int N, int E>
 __global__ void __launch_bounds__(256)
-    k_coded_y_rows(float2* __restrict__ tb, const float2* __restrict__ xfwd, const float2* __restrict__ mask,
-                   int64_t H, int64_t B, int chunk, int ngroups, float inv_n, float2* __restrict__ gc_part) {
+    k_coded_y_rows(const float2* __restrict__ tb, float2* __restrict__ out, const float2* __restrict__ xfwd,
+                   const float2* __restrict__ mask, int64_t H, int64_t B, int chunk, int ngroups, float inv_n,
+                   float2* __restrict__ gc_part) {
   // Views of a chunk are pipelined: the next view's row (T) is loaded while
   // this view's transforms run, and its X row once this view's X is consumed;
   // buffer addressing (one VGPR per lane), LDS-only barriers and LDS twiddles
@@ -323,11 +338,7 @@ __global__ void __launch_bounds__(256)
     if (b + 1 < b1) fetch_x(b + 1, nx);
     lds_barrier();
     fft_seq_tw<N, E, -1>(v, tt, sl, s_tw);
-    if (yv) {
-      const auto r = buffer_rsrc(tb + b * H * N, fbytes);
-#pragma unroll
-      for (int q = 0; q < E; ++q) buf_store_f2(v[q], r, lane, (uint32_t)(q * T * 8));
-    }
+    if (yv) store_row_strip<N, E>(out + b * H * N, (uint32_t)H, (uint32_t)y, t, v);  // column-strip layout
     lds_barrier();
   }
   if (yv) {
@@ -336,7 +347,8 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Q4: per (column tile, chunk of views): column FFT; spectrum partial
+// Q4: per (column tile, chunk of views; frames in the column-strip layout):
+// column FFT; spectrum partial
 // += conj(H1 R_b) v / N, with the view-independent conj(H1)/N applied once
 // after the view loop.  About one work-group per CU (ColChunkQuery), each on
 // one column tile and a chunk of consecutive views, software-pipelined like
@@ -369,9 +381,11 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   const unsigned b0 = ch * (unsigned)chunk;                            // < B
   const unsigned b1 = b0 + chunk < (unsigned)B ? b0 + chunk : (unsigned)B;
   const uint32_t fbytes = (uint32_t)(N * width * 8);
-  const uint32_t lane = (uint32_t)((t * width + c) * 8);
   const uint32_t col0 = (uint32_t)(tile * C * 8);
-  const uint32_t rstep = (uint32_t)(T * width * 8);
+  // the frames are in the column-strip layout: the tile is a contiguous block
+  const uint32_t lg = strip_lg(width);
+  const uint32_t sbase = strip_off(0u, (uint32_t)(tile * C), (uint32_t)N, lg);
+  const uint32_t lane = (strip_off((uint32_t)t, (uint32_t)col, (uint32_t)N, lg) - sbase) * 8u;
   float2 acc[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) acc[q] = c_zero();
@@ -385,7 +399,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     for (int q = 0; q < E; ++q) dy[q] = buf_load_f2(ry, (uint32_t)t * 8u, (uint32_t)(q * T * 8));
     const auto r = buffer_rsrc(tb + (int64_t)bb * N * width, live ? fbytes : 0u);
 #pragma unroll
-    for (int q = 0; q < E; ++q) dv[q] = buf_load_f2(r, lane, q * rstep + col0);
+    for (int q = 0; q < E; ++q) dv[q] = buf_load_f2(r, lane, (sbase + ((uint32_t)(q * T) << lg)) * 8u);
   };
   auto process = [&](float2 (&v)[E], const float2 (&ry)[E], float2 rx, bool second) {
     // the exchange addresses depend only on (t, c): recomputed per view from the
@@ -533,17 +547,19 @@ struct SpecColsRun {
 };
 template <int N, int E>
 struct XRowsRun {
-  static ptx_status run(float2* tb, float2* xfwd, const float2* mask, int64_t H, int64_t nrows, float inv_n,
-                        hipStream_t s) {
+  static ptx_status run(const float2* tb, float2* out, float2* xfwd, const float2* mask, int64_t H, int64_t nrows,
+                        float inv_n, hipStream_t s) {
     const int64_t blocks = (nrows + RowCfg<N, E>::S - 1) / RowCfg<N, E>::S;
-    k_coded_x_rows<N, E><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(tb, xfwd, mask, H, nrows, inv_n);
+    k_coded_x_rows<N, E><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(tb, out, xfwd, mask, H, nrows, inv_n);
     return check_launch("coded x rows", s);
   }
 };
+// src (column-strip layout) -> tb (plain layout)
 template <int N, int E>
 struct TransferRun {
-  static ptx_status run(float2* tb, const float2* tr, int64_t frames, int64_t width, bool conj, hipStream_t s) {
-    launch_transfer_cols<N, E>(tb, tr, frames, width, conj, s);
+  static ptx_status run(float2* tb, const float2* src, const float2* tr, int64_t frames, int64_t width, bool conj,
+                        hipStream_t s) {
+    launch_transfer_cols<N, E>(tb, tr, frames, width, conj, s, src);
     return check_launch(conj ? "coded transfer* cols" : "coded transfer cols", s);
   }
 };
@@ -563,11 +579,11 @@ struct GradZRowsRun {
 };
 template <int N, int E>
 struct StepRowsRun {
-  static ptx_status run(float2* tb, const float* measured, float* amp, float* loss_part, int64_t nrows, float gscale,
-                        float inv_n, hipStream_t s) {
+  static ptx_status run(const float2* tb, float2* out, const float* measured, float* amp, float* loss_part, int64_t H,
+                        int64_t nrows, float gscale, float inv_n, hipStream_t s) {
     const int64_t blocks = (nrows + RowCfg<N, E>::S - 1) / RowCfg<N, E>::S;
-    k_coded_step_rows<N, E><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(tb, measured, amp, loss_part, nrows, gscale,
-                                                                      inv_n);
+    k_coded_step_rows<N, E><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(tb, out, measured, amp, loss_part, H, nrows,
+                                                                      gscale, inv_n);
     return check_launch("coded step rows", s);
   }
 };
@@ -605,10 +621,10 @@ struct ColChunkQuery {
 };
 template <int N, int E>
 struct YRowsRun {
-  static ptx_status run(float2* tb, const float2* xfwd, const float2* mask, int64_t H, int64_t B, const Chunks& c,
-                        float inv_n, float2* gc_part, hipStream_t s) {
+  static ptx_status run(const float2* tb, float2* out, const float2* xfwd, const float2* mask, int64_t H, int64_t B,
+                        const Chunks& c, float inv_n, float2* gc_part, hipStream_t s) {
     k_coded_y_rows<N, E><<<dim3((unsigned)(c.row_groups * c.row_nchunks)), dim3(256), 0, s>>>(
-        tb, xfwd, mask, H, B, c.row_chunk, c.row_groups, inv_n, gc_part);
+        tb, out, xfwd, mask, H, B, c.row_chunk, c.row_groups, inv_n, gc_part);
     return check_launch("coded y rows", s);
   }
 };
@@ -689,6 +705,7 @@ struct CodedWorkspace {
   float2* ey;       // [B, H]
   float2* xfwd;     // [B, H, W] (step)
   float2* tb;       // [B, H, W]
+  float2* strip;    // [B, H, W] in the column-strip layout (inputs of the column passes)
   float2* gc_part;  // [row chunks, H, W]
   float2* gs_part;  // [col chunks, H, W]
   float2* gspec;    // [H, W]
@@ -705,6 +722,7 @@ CodedWorkspace carve(void* base, int64_t H, int64_t W, int64_t B, int64_t ds, bo
   ws.ey = cv.take<float2>((size_t)(B * H));
   ws.xfwd = step ? cv.take<float2>((size_t)(B * H * W)) : nullptr;
   ws.tb = cv.take<float2>((size_t)(B * H * W));
+  ws.strip = cv.take<float2>((size_t)(B * H * W));
   ws.gc_part = cv.take<float2>((size_t)(c.row_nchunks * H * W));
   ws.gs_part = cv.take<float2>((size_t)(c.col_nchunks * H * W));
   ws.gspec = cv.take<float2>((size_t)(H * W));
@@ -745,18 +763,19 @@ ptx_status ramps(const float* shifts, int64_t B, int64_t H, int64_t W, const Cod
   return check_launch("coded ramp tables", s);
 }
 
-// P1..P3 of the forward: spectrum -> X (stored) -> T holds F_cols^-1 (F(X C) H2) row-transformed
+// P1..P3 of the forward: spectrum -> X (stored) -> tb holds F_cols^-1 (F(X C) H2) row-transformed
 ptx_status forward_to_t(const float2* spec, const float2* mask, const float2* t1, const float2* t2, int64_t H,
                         int64_t W, int64_t B, const CodedWorkspace& ws, float2* xfwd, hipStream_t s) {
   const float inv_n = 1.0f / (float)(H * W);
   CodedSpecCols p1{spec, t1, ws.ex, ws.ey, ws.tb, H, W};
   ptx_status st = dispatch_ne<SpecColsRun>(PASS_COL, H, p1, B, s);
   if (st != PTX_OK) return st;
-  if ((st = dispatch_ne<XRowsRun>(PASS_ROW, W, ws.tb, xfwd, mask, H, B * H, inv_n, s)) != PTX_OK) return st;
-  return dispatch_ne<TransferRun>(PASS_COL, H, ws.tb, t2, B, W, false, s);
+  if ((st = dispatch_ne<XRowsRun>(PASS_ROW, W, ws.tb, ws.strip, xfwd, mask, H, B * H, inv_n, s)) != PTX_OK) return st;
+  return dispatch_ne<TransferRun>(PASS_COL, H, ws.tb, ws.strip, t2, B, W, false, s);
 }
 
-// Q2..end: T holds row-FFT of gZ; produces grad_object and grad_coded.
+// Q2..end: strip holds the row FFT of gZ (column-strip layout); produces
+// grad_object and grad_coded.
 // matlab: both gradients conjugated, as the MATLAB near-field backward returns
 // them -- grad_coded = conj(sum_b gY conj(X_b)) and
 // grad_object = conj(N ifft2(S)) = fft2(conj(S)) for the spectrum gradient S
@@ -766,10 +785,13 @@ ptx_status backward_from_t(const float2* xfwd, const float2* mask, const float2*
                            int64_t W, int64_t B, const Chunks& c, const CodedWorkspace& ws, float2* gobj,
                            float2* gcoded, int accumulate, hipStream_t s, bool matlab = false) {
   const float inv_n = 1.0f / (float)(H * W);
-  ptx_status st = dispatch_ne<TransferRun>(PASS_COL, H, ws.tb, t2, B, W, true, s);
+  ptx_status st = dispatch_ne<TransferRun>(PASS_COL, H, ws.tb, ws.strip, t2, B, W, true, s);
   if (st != PTX_OK) return st;
-  if ((st = dispatch_ne<YRowsRun>(PASS_VIEW_ROWS, W, ws.tb, xfwd, mask, H, B, c, inv_n, ws.gc_part, s)) != PTX_OK) return st;
-  if ((st = dispatch_ne<SpecGradColsRun>(PASS_COL_LOOP, H, ws.tb, t1, ws.ex, ws.ey, B, W, c, inv_n, ws.gs_part, s)) != PTX_OK)
+  if ((st = dispatch_ne<YRowsRun>(PASS_VIEW_ROWS, W, ws.tb, ws.strip, xfwd, mask, H, B, c, inv_n, ws.gc_part, s)) !=
+      PTX_OK)
+    return st;
+  if ((st = dispatch_ne<SpecGradColsRun>(PASS_COL_LOOP, H, ws.strip, t1, ws.ex, ws.ey, B, W, c, inv_n, ws.gs_part,
+                                         s)) != PTX_OK)
     return st;
   sum_parts2(ws.gc_part, c.row_nchunks, H * W, gcoded, accumulate, s, matlab ? 1 : 0);
   if ((st = check_launch("mask partial sum", s)) != PTX_OK) return st;
@@ -877,7 +899,7 @@ ptx_status ptx_coded_backward(const float* grad_output, const void* object_spect
     if ((st = check_launch("coded pool", s)) != PTX_OK) return st;
     scmap = ws.sc;
   }
-  CodedGradZRows q1{(const float2*)sensor_field, scmap, ws.tb, H, (int)downsample, matlab ? 1 : 0};
+  CodedGradZRows q1{(const float2*)sensor_field, scmap, ws.strip, H, (int)downsample, matlab ? 1 : 0};
   if ((st = dispatch_ne<GradZRowsRun>(PASS_ROW, W, q1, B * H, s)) != PTX_OK) return st;
   return backward_from_t((const float2*)x_forward, (const float2*)coded_surface, (const float2*)transfer_1,
                          (const float2*)transfer_2, H, W, B, c, ws, (float2*)grad_object, (float2*)grad_coded, acc, s,
@@ -926,8 +948,8 @@ ptx_status ptx_coded_step(const void* object_field, int64_t H, int64_t W, const 
     return st;
   const float inv_n = 1.0f / (float)(H * W);
   if (ds == 1) {
-    if ((st = dispatch_ne<StepRowsRun>(PASS_ROW, W, ws.tb, measured, amplitude, ws.loss_part, B * H, 2.0f * loss_scale, inv_n,
-                                      s)) != PTX_OK)
+    if ((st = dispatch_ne<StepRowsRun>(PASS_LOSS_ROWS, W, ws.tb, ws.strip, measured, amplitude, ws.loss_part, H, B * H,
+                                       2.0f * loss_scale, inv_n, s)) != PTX_OK)
       return st;
     loss_rows(ws.loss_part, H, B, inv_n, loss_per_position, s);
     if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
@@ -941,7 +963,7 @@ ptx_status ptx_coded_step(const void* object_field, int64_t H, int64_t W, const 
     if ((st = check_launch("coded pool", s)) != PTX_OK) return st;
     loss_rows(ws.loss_part, lh * lw, B, 1.0f / (float)(lh * lw), loss_per_position, s);
     if ((st = check_launch("loss rows", s)) != PTX_OK) return st;
-    CodedGradZRows q1{ws.tb, ws.sc, ws.tb, H, (int)ds, 0};
+    CodedGradZRows q1{ws.tb, ws.sc, ws.strip, H, (int)ds, 0};
     if ((st = dispatch_ne<GradZRowsRun>(PASS_ROW, W, q1, B * H, s)) != PTX_OK) return st;
   }
   return backward_from_t(ws.xfwd, (const float2*)coded_surface, (const float2*)transfer_1, (const float2*)transfer_2,

@@ -30,7 +30,7 @@ inline bool size_ok(int64_t n) { return is_pow2(n) && n >= PTX_MIN_N && n <= PTX
 ptx_status check_launch(const char* what, hipStream_t stream = nullptr);
 
 // positions_out[b] = positions_all[order[b]]; an order entry outside [0, count)
-// reads entry 0 instead and sets *bad (defined in ptx_optim.hip).
+// reads entry 0 instead and sets *bad when bad is non-null (defined in ptx_optim.hip).
 ptx_status launch_gather_positions(const int64_t* order, const int64_t* positions_all, int64_t count, int64_t batch,
                                    int64_t* positions_out, int* bad, hipStream_t stream);
 
@@ -53,7 +53,17 @@ void timing_begin(hipStream_t stream);
 // Pass kinds with their own elements-per-thread setting in the tuning table.
 // PASS_VIEW_ROWS: row passes that loop over views holding per-element state
 // of several frames in registers (the coded mask-gradient pass).
-enum PassKind { PASS_ROW = 0, PASS_ROW_LOOP = 1, PASS_COL = 2, PASS_COL_LOOP = 3, PASS_VIEW_ROWS = 4, PASS_KINDS = 5 };
+// PASS_LOSS_ROWS: row passes that evaluate the amplitude loss between an
+// inverse and a forward transform (the coded step's sensor-plane pass).
+enum PassKind {
+  PASS_ROW = 0,
+  PASS_ROW_LOOP = 1,
+  PASS_COL = 2,
+  PASS_COL_LOOP = 3,
+  PASS_VIEW_ROWS = 4,
+  PASS_LOSS_ROWS = 5,
+  PASS_KINDS = 6
+};
 
 // Elements per thread (8 or 16) the tuning table holds for (kind, n); n a
 // supported power of two.  Defaults come from MI355X measurements (radix-8

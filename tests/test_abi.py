@@ -185,7 +185,7 @@ def test_group_table_roundtrip(lib):
     lib.call("ptx_tune_set_groups", 1, 512, 0)
     assert h.ptx_tune_get_groups(1, 512) == 0
     with pytest.raises(ValueError):
-        lib.call("ptx_tune_set_groups", 5, 512, 64)
+        lib.call("ptx_tune_set_groups", 6, 512, 64)
     with pytest.raises(ValueError):
         lib.call("ptx_tune_set_groups", 0, 512, -1)
     with pytest.raises(ValueError):
