@@ -66,8 +66,9 @@ def recorded_traffic(kernel: str):
         return None, "stale: the kernel's sources changed since the profile"
     return rec["bytes"], "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/base_step_obj2048_probe512_b64_pmc_traffic.md"
 
-# Stated latency bound of the served mix: p99 from arrival to completion at
-# every measured Poisson rate up to 0.9x the server's saturation throughput.
+# Latency bound of the served mix: p99 from arrival to completion; the bench
+# reports the highest measured load (fraction of the server's saturation
+# throughput) at which every Poisson arrival sample stayed within it.
 P99_BOUND_MS = 100.0
 
 
@@ -463,20 +464,30 @@ def measure_serving(dev, args) -> dict:
                                     "class's difference (the plain path fed the fused first-epoch gradients ends "
                                     "within 1e-5 of the fused final loss)"}
     print(f"[bench] plain per-request path: {plain_sat}", file=sys.stderr, flush=True)
-    for frac, seed in ((0.5, 12), (0.9, 13)):
+    rates[0]["fraction_of_saturation"] = round(low / sat["throughput_req_per_s"], 3)
+    # the server alone at fixed fractions of its saturation rate; at the higher
+    # loads p99 (the 4th-largest of 400 latencies) depends on the arrival
+    # sample, so each of those rates runs several Poisson samples
+    for frac, seeds in ((0.5, (12,)), (0.75, (14, 15, 16)), (0.9, (13, 17, 18))):
         rate = frac * sat["throughput_req_per_s"]
-        rates.append({"rate_req_per_s": round(rate, 3), "server": sv.summarize(sv.Server(dev).run(arrivals(rate, seed),
-                                                                                              probs)),
-                      "plain": None})
+        for seed in seeds:
+            rates.append({"rate_req_per_s": round(rate, 3), "fraction_of_saturation": frac, "arrival_seed": seed,
+                          "server": sv.summarize(sv.Server(dev).run(arrivals(rate, seed), probs)), "plain": None})
     for r in rates:
         print(f"[bench] serving at {r['rate_req_per_s']} req/s: server {r['server']}"
               + (f", plain {r['plain']}" if r["plain"] else ""), file=sys.stderr, flush=True)
-    held = all(r["server"]["p99_ms"] <= P99_BOUND_MS for r in rates)
-    print(f"[bench] serving p99 bound {P99_BOUND_MS} ms at rates up to 0.9x saturation: "
-          f"{'held' if held else 'NOT held'}", file=sys.stderr, flush=True)
+    # the highest measured load at which every arrival sample kept p99 within the bound
+    held_up_to = 0.0
+    for frac in sorted({r["fraction_of_saturation"] for r in rates}):
+        if all(r["server"]["p99_ms"] <= P99_BOUND_MS for r in rates if r["fraction_of_saturation"] == frac):
+            held_up_to = frac
+        else:
+            break
+    print(f"[bench] serving p99 <= {P99_BOUND_MS} ms in every arrival sample up to {held_up_to}x saturation",
+          file=sys.stderr, flush=True)
     return {
         "p99_bound_ms": P99_BOUND_MS,
-        "p99_bound_held": held,
+        "p99_bound_held_up_to_fraction_of_saturation": held_up_to,
         "workload": "mixed stream: conventional 128/512 and 256/1024 (100-400 positions), Fourier 128^2/512^2 "
                     "(81|225 LEDs), near-field 512/1024 (64), coded 512^2 (25); 20 Adam epochs each",
         "requests": n,
