@@ -73,6 +73,36 @@ def recorded_traffic(kernel: str):
 P99_BOUND_MS = 100.0
 
 
+def measure_copy_peak(dev, nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Read + write bytes per second (GB/s) of the library's streaming copy
+    (16-byte loads and stores) between two 1 GiB buffers: the rate plain
+    mixed traffic reaches from HBM on this device, the practical ceiling of
+    the fused passes (the nominal 8 TB/s counts one direction at a time)."""
+    import torch
+
+    from ptychox_amd import _lib
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    s = _lib.stream_of(dev)
+
+    def copy():
+        _lib.call("ptx_stream_copy", _lib.ptr(a), _lib.ptr(b), nbytes, s)
+
+    for _ in range(3):
+        copy()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        copy()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return round(2 * nbytes / (ms * 1e-3) / 1e9, 1)
+
+
 def parse_args(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +110,7 @@ def parse_args(argv=None):
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-copy-peak", action="store_true", help="skip the streaming-copy peak measurement")
     ap.add_argument("--no-plain", action="store_true", help="skip the unfused torch path")
     ap.add_argument("--atomic", action="store_true",
                     help="object gradient by float atomics instead of the (default) sorted-segment gather")
@@ -412,6 +443,9 @@ def measure_serving(dev, args) -> dict:
     t0 = time.perf_counter()
     with ThreadPoolExecutor(8) as ex:
         probs = dict(zip([r.rid for r in base], ex.map(sv.make_problem, base)))
+    # the request data as a receiving server holds it: page-locked host memory
+    # (both paths upload from it)
+    probs = {rid: sv.pin_problem(p) for rid, p in probs.items()}
     print(f"[bench] serving: {n} requests synthesised in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
           flush=True)
 
@@ -597,6 +631,12 @@ def main(argv=None) -> int:
     for label, d in kernels.items():
         print(f"[bench]   {label:22s} {d['avg_ms']:.4f} ms  {d['gbs']} GB/s", file=sys.stderr, flush=True)
 
+    # the HBM rate a plain streaming copy reaches here (read + write bytes per second)
+    copy_peak = None if args.no_copy_peak else measure_copy_peak(dev)
+    if copy_peak:
+        print(f"[bench] streaming copy peak {copy_peak:.0f} GB/s; dominant kernel at "
+              f"{dom_gbs / copy_peak:.3f} of it", file=sys.stderr, flush=True)
+
     # whole-path algorithmic bytes per pattern: B_io (forward patch 8N, adjoint
     # patch 8N, amplitude 4N, gradient-of-amplitude 4N, object RMW 16N) + one
     # 16N intermediate round trip per LDS-exceeding 2-D FFT (2 per pattern)
@@ -683,7 +723,9 @@ def main(argv=None) -> int:
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
                      "traffic": recorded_traffic(dom)[0], "traffic_source": recorded_traffic(dom)[1],
-                     "algorithmic_bytes": int(dom_bytes)},
+                     "algorithmic_bytes": int(dom_bytes),
+                     "measured_copy_peak": copy_peak,
+                     "frac_of_copy_peak": round(dom_gbs / copy_peak, 4) if copy_peak else None},
         "path_roofline": {"bytes_per_pattern": b_alg, "achieved": round(path_gbs, 1), "unit": "GB/s",
                           "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,

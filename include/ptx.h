@@ -100,6 +100,14 @@ ptx_status ptx_timing_events(void* const* events, int n);
 int ptx_timing_count(void);
 const char* ptx_timing_label(int k);
 
+/* Streaming copy of ``bytes`` bytes from src to dst (device pointers, 16-byte
+ * aligned, bytes a multiple of 16, the ranges not overlapping) on ``stream``:
+ * 16-byte loads and stores, four per thread in flight, a grid of up to 16384
+ * work-groups striding over the buffer.  No counterpart in the reference: it
+ * measures the HBM rate a plain read + write stream reaches on this device,
+ * the practical ceiling the fused passes' mixed traffic is compared with. */
+ptx_status ptx_stream_copy(const void* src, void* dst, size_t bytes, void* stream);
+
 /* ------------------------------------------------------------------ tuning table
  * No counterpart in the reference (cuFFT chose its own kernels).  Every FFT pass
  * is compiled for 8 and for 16 complex elements per thread (the radix of its

@@ -59,6 +59,7 @@ SIGNATURES: dict[str, tuple] = {
     "ptx_tune_get": (_i, [_i, _i64]),
     "ptx_tune_set_groups": (_st, [_i, _i64, _i]),
     "ptx_tune_get_groups": (_i, [_i, _i64]),
+    "ptx_stream_copy": (_st, [_p, _p, _sz, _p]),
     "ptx_fft2d": (_st, [_p, _i64, _i64, _i64, _i, _p]),
     "ptx_base_forward": (_st, [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _p, _p]),
     "ptx_base_backward_workspace": (_sz, [_i64, _i64, _i64, _i64, _i64, _u]),

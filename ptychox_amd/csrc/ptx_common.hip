@@ -1,5 +1,6 @@
 // Error reporting, version queries and the standalone batched 2-D FFT entry point.
 #include <atomic>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdio>
 
@@ -98,6 +99,27 @@ int device_cu_count() {
   return n;
 }
 
+// ---------------------------------------------------------------- streaming copy
+// Grid-stride copy of n 16-byte elements: thread t of chunk c moves elements
+// c * 1024 + t + 256 u (u < 4), all four loads issued before the stores; the
+// chunk where a thread's last element passes n is finished with bounds checks.
+__global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b,
+                                                     size_t n) {
+  constexpr int U = 4;
+  const size_t stride = (size_t)gridDim.x * 256 * U;
+  size_t i = (size_t)blockIdx.x * 256 * U + threadIdx.x;
+  for (; i + 256 * (U - 1) < n; i += stride) {
+    float4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = a[i + 256 * u];
+#pragma unroll
+    for (int u = 0; u < U; ++u) b[i + 256 * u] = v[u];
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (i + 256 * u < n) b[i + 256 * u] = a[i + 256 * u];
+}
+
 template <int N, int E>
 struct FftRowsRun {
   static ptx_status run(float2* data, int64_t nrows, int inverse, hipStream_t s) {
@@ -186,6 +208,25 @@ ptx_status ptx_tune_set_groups(int kind, int64_t n, int groups) {
 int ptx_tune_get_groups(int kind, int64_t n) { return size_ok(n) ? tuned_groups(kind, n, 0) : 0; }
 
 const char* ptx_timing_label(int k) { return (k >= 0 && k < t_used) ? t_labels[k] : ""; }
+
+ptx_status ptx_stream_copy(const void* src, void* dst, size_t bytes, void* stream) {
+  if (bytes == 0) return PTX_OK;
+  if (!src || !dst || bytes % 16 != 0 || ((uintptr_t)src | (uintptr_t)dst) % 16 != 0) {
+    set_error("ptx_stream_copy: pointers must be non-null and 16-byte aligned, bytes a multiple of 16");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  const uintptr_t s0 = (uintptr_t)src, d0 = (uintptr_t)dst;
+  if (s0 < d0 + bytes && d0 < s0 + bytes) {
+    set_error("ptx_stream_copy: source and destination overlap");
+    return PTX_ERR_INVALID_ARGUMENT;
+  }
+  const size_t n = bytes / 16;
+  size_t grid = (n + 1023) / 1024;
+  if (grid > 16384) grid = 16384;
+  hipStream_t s = (hipStream_t)stream;
+  k_stream_copy<<<dim3((unsigned)grid), dim3(256), 0, s>>>((const float4*)src, (float4*)dst, n);
+  return check_launch("stream copy", s);
+}
 
 const char* ptx_version(void) { return "ptychox_hip 0.1.0 (gfx950)"; }
 

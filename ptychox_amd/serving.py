@@ -180,6 +180,13 @@ def make_problem(req: Request) -> Problem:
     })
 
 
+def pin_problem(prob: Problem) -> Problem:
+    """The same request data in page-locked host memory, as a receiving
+    server would hold it: the admission's uploads then run asynchronously on
+    the request's stream instead of blocking the serving loop."""
+    return Problem(prob.kind, {k: v.pin_memory() for k, v in prob.arrays.items()})
+
+
 def _workspace_bytes(req: Request) -> int:
     s = shapes(req)
     (H, W), (h, w), V = s["object"], s["aux"], s["views"]
@@ -240,11 +247,33 @@ def hyper_table(lr: float, epochs: int) -> torch.Tensor:
     return torch.tensor(rows + [[0.0, 1.0, 1.0, 0.0]], dtype=torch.float32)
 
 
-def epoch_hyper(table: torch.Tensor, counter: torch.Tensor, epochs: int) -> torch.Tensor:
+def epoch_hyper(table: torch.Tensor, counter: torch.Tensor, epochs: int,
+                out: tuple[torch.Tensor, torch.Tensor] | None = None) -> torch.Tensor:
     """The row of ``table`` for the epoch ``counter`` (a one-element int64
     device tensor) counts, selected on the device; the index is clamped to
-    the last row, so a counter past the plan never reads beyond the table."""
-    return table.index_select(0, counter.clamp_max(epochs))[0]
+    the last row, so a counter past the plan never reads beyond the table.
+    ``out`` = (index [1] int64, row [1, 4] float32) preallocated on the
+    table's device: the selection then allocates nothing (a captured epoch
+    that allocates nothing leaves its graph no memory pool to free)."""
+    if out is None:
+        return table.index_select(0, counter.clamp_max(epochs))[0]
+    idx, row = out
+    torch.clamp(counter, max=epochs, out=idx)
+    torch.index_select(table, 0, idx, out=row)
+    return row[0]
+
+
+_HYPER_CACHE: dict = {}
+
+
+def device_hyper_table(lr: float, epochs: int, device: torch.device) -> torch.Tensor:
+    """``hyper_table(lr, epochs)`` on ``device``, built once per (lr, epochs,
+    device) and shared read-only by the requests that use it."""
+    key = (float(lr), int(epochs), str(device))
+    t = _HYPER_CACHE.get(key)
+    if t is None:
+        t = _HYPER_CACHE[key] = hyper_table(lr, epochs).to(device)
+    return t
 
 
 class Job:
@@ -265,7 +294,9 @@ class Job:
         self.done_event: torch.cuda.Event | None = None
         before = torch.cuda.memory_allocated(device)
         with torch.cuda.stream(stream):
-            a = {k: v.to(device) for k, v in prob.arrays.items()}
+            # asynchronous on the request's stream when the host arrays are
+            # pinned (pin_problem), so admission does not wait for the upload
+            a = {k: v.to(device, non_blocking=True) for k, v in prob.arrays.items()}
             self.obj = a.pop("object")
             self.aux = a.pop("aux")
             self.a = a
@@ -284,8 +315,13 @@ class Job:
                 self.step = NearFieldStep(self.obj.shape, self.aux.shape, V, device, scatter="segments")
             else:
                 self.step = CodedStep(self.obj.shape, V, 1, device)
-            self.hyper_table = hyper_table(req.lr, req.epochs).to(device)
+            self.hyper_table = device_hyper_table(req.lr, req.epochs, device)
             self.counter = torch.zeros(1, dtype=torch.int64, device=device)
+            self._hyper_out = (torch.empty(1, dtype=torch.int64, device=device),
+                               torch.empty((1, 4), dtype=torch.float32, device=device))
+            self._loss_dev = torch.empty((), dtype=torch.float32, device=device)
+        # the final loss, copied to pinned host memory behind the last epoch
+        self._loss_host = torch.empty((), dtype=torch.float32, pin_memory=True)
         self.resident_bytes = torch.cuda.memory_allocated(device) - before
 
     def _fused_step(self) -> None:
@@ -303,7 +339,7 @@ class Job:
         # table has epochs + 1 rows (the last one all zeros); the index is
         # clamped on the device too, so even a replay past the planned epochs
         # reads the zero row instead of memory past the table.
-        hyper = epoch_hyper(self.hyper_table, self.counter, self.req.epochs)
+        hyper = epoch_hyper(self.hyper_table, self.counter, self.req.epochs, self._hyper_out)
         self._fused_step()
         self.opt_o.step(self.g_obj, hyper)
         self.opt_a.step(self.g_aux, hyper)
@@ -334,6 +370,11 @@ class Job:
                 if self.graph is None:
                     self._capture()
                 self.graph.replay()
+            if self.launched + 1 == self.req.epochs:
+                # the final loss travels to the host behind the last epoch, so
+                # reading it after completion needs no device synchronisation
+                torch.mean(self.step.loss[: self.views], dim=0, out=self._loss_dev)
+                self._loss_host.copy_(self._loss_dev, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(self.stream)
         self.inflight.append(ev)
@@ -357,8 +398,16 @@ class Job:
 
     def final_loss(self) -> float:
         """Amplitude MSE at the last epoch's forward pass (the per-view losses
-        are means over each frame's pixels)."""
-        return float(self.step.loss[: self.views].mean())
+        are means over each frame's pixels); read once the request has
+        finished (its copy to the host is done then)."""
+        if not self.finished:
+            raise RuntimeError(f"request {self.req.rid} has not finished")
+        return float(self._loss_host)
+
+    def take_graph(self) -> torch.cuda.CUDAGraph | None:
+        """Detach the captured epoch (to be destroyed later by the caller)."""
+        g, self.graph = self.graph, None
+        return g
 
     def release(self) -> None:
         """Drop the graph and every device buffer now (all queued work is done)."""
@@ -366,7 +415,7 @@ class Job:
         self.step = self.opt_o = self.opt_a = None
         self.obj = self.aux = self.g_obj = self.g_aux = None
         self.a = {}
-        self.hyper_table = self.counter = None
+        self.hyper_table = self.counter = self._hyper_out = self._loss_dev = None
         self.inflight.clear()
         self.done_event = None
 
@@ -394,7 +443,7 @@ class Server:
     and returns one :class:`Completed` record per request, in request order."""
 
     def __init__(self, device="cuda", hbm_budget: float | None = None, slots: int = 8, graph: bool = True,
-                 depth: int = 2, keep_results: bool = False):
+                 depth: int = 2, keep_results: bool = False, retire_batch: int = 32):
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -404,6 +453,11 @@ class Server:
         self.budget = int(hbm_budget)
         self.slots, self.graph, self.depth = int(slots), bool(graph), int(depth)
         self.keep = bool(keep_results)
+        # Destroying a graph waits for the work in flight on the device, so
+        # finished requests' graphs are destroyed together: when this many
+        # have piled up, or whenever no request is running.  Their captures
+        # allocate nothing, so a retired graph holds no device buffers.
+        self.retire_batch = max(int(retire_batch), 1)
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(self.slots)]
         self.peak_in_flight = 0
         self.peak_bytes = 0
@@ -421,6 +475,7 @@ class Server:
         free_slots = list(range(self.slots))
         used = 0
         done: list[Completed] = []
+        retired: list = []  # graphs of finished requests, destroyed in batches
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
         while pending or queue or active:
@@ -439,6 +494,7 @@ class Server:
                                         job.final_loss(), nbytes, job.resident_bytes)
                         if self.keep:
                             rec.object, rec.aux = job.obj.clone(), job.aux.clone()
+                    retired.append(job.take_graph())
                     job.release()
                     done.append(rec)
                     del active[slot]
@@ -454,6 +510,8 @@ class Server:
                 job = Job(req, problems[req.rid], self.device, self.streams[slot], self.graph, self.depth)
                 active[slot] = (job, nbytes, time.perf_counter() - t0)
                 used += nbytes
+            if retired and (len(retired) >= self.retire_batch or not active):
+                retired.clear()
             self.peak_in_flight = max(self.peak_in_flight, len(active))
             self.peak_bytes = max(self.peak_bytes, used)
             progressed = False

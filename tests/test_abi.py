@@ -66,6 +66,17 @@ def test_unsupported_sizes_are_rejected_without_gpu(lib):
     assert st == 1 and b"null" in h.ptx_last_error()
 
 
+def test_stream_copy_rejects_bad_arguments_without_gpu(lib):
+    h = lib.lib()
+    assert h.ptx_stream_copy(None, None, 0, None) == 0  # nothing to copy
+    st = h.ptx_stream_copy(ctypes.c_void_p(16), ctypes.c_void_p(4096), 24, None)
+    assert st == 1 and b"multiple of 16" in h.ptx_last_error()
+    st = h.ptx_stream_copy(ctypes.c_void_p(8), ctypes.c_void_p(4096), 32, None)
+    assert st == 1 and b"aligned" in h.ptx_last_error()
+    st = h.ptx_stream_copy(ctypes.c_void_p(4096), ctypes.c_void_p(4096 + 32), 64, None)
+    assert st == 1 and b"overlap" in h.ptx_last_error()
+
+
 def test_workspace_queries(lib):
     n = lib.query("ptx_base_backward_workspace", 2048, 2048, 512, 512, 64, 0)
     assert n >= 64 * 512 * 512 * 8

@@ -26,6 +26,17 @@ def _lib():
     return _lib
 
 
+@pytest.mark.parametrize("nbytes", [16, 4096 * 16 + 48, 64 << 20])
+def test_stream_copy_is_exact(gpu, nbytes):
+    lib = _lib()
+    src = torch.randint(-(2**31), 2**31 - 1, (nbytes // 4,), dtype=torch.int32, device=gpu)
+    dst = torch.full((nbytes // 4 + 4,), 7, dtype=torch.int32, device=gpu)  # 16 guard bytes past the end
+    lib.call("ptx_stream_copy", lib.ptr(src), lib.ptr(dst), nbytes, lib.stream_of(gpu))
+    torch.cuda.synchronize()
+    assert torch.equal(dst[: nbytes // 4], src)
+    assert torch.all(dst[nbytes // 4 :] == 7)
+
+
 @pytest.mark.parametrize("h,w", [(8, 8), (16, 64), (64, 64), (128, 32), (256, 256), (512, 512), (1024, 128), (2048, 16)])
 @pytest.mark.parametrize("inverse", [0, 1])
 def test_fft2d_matches_torch(gpu, h, w, inverse):
