@@ -338,7 +338,7 @@ def measure_epoch(ds, dev, batch: int = BATCH, label: str = "900 positions") -> 
     return out
 
 
-def measure_modalities(dev, args) -> dict:
+def measure_modalities(dev, args, copy_peak=None) -> dict:
     """Fused-step throughput of the other three operators at their full sizes,
     beside the unfused torch path on the same GPU."""
     import torch
@@ -361,7 +361,9 @@ def measure_modalities(dev, args) -> dict:
         frac = B_ALG[name] * v / (HBM_PEAK_GBS * 1e9)
         out[name] = {"value": round(v, 1), "unit": "patterns/s", "ms_per_step": round(ms, 4),
                      "plain_value": round(B / (pms / 1e3), 1), "speedup_of_fused": round(pms / ms, 2),
-                     "bytes_per_pattern": B_ALG[name], "path_roofline_frac": round(frac, 4)}
+                     "bytes_per_pattern": B_ALG[name], "path_roofline_frac": round(frac, 4),
+                     "path_frac_of_copy_peak": (round(B_ALG[name] * v / (copy_peak * 1e9), 4)
+                                                if copy_peak else None)}
         print(f"[bench] {name}: {ms:.3f} ms/step, {v:.0f} patterns/s ({frac:.1%} of HBM roofline), "
               f"plain {pms:.3f} ms", file=sys.stderr, flush=True)
 
@@ -640,7 +642,8 @@ def main(argv=None) -> int:
     # whole-path algorithmic bytes per pattern: B_io (forward patch 8N, adjoint
     # patch 8N, amplitude 4N, gradient-of-amplitude 4N, object RMW 16N) + one
     # 16N intermediate round trip per LDS-exceeding 2-D FFT (2 per pattern)
-    b_alg = (8 + 8 + 4 + 4 + 16) * N + 2 * 16 * N
+    b_io = (8 + 8 + 4 + 4 + 16) * N
+    b_alg = b_io + 2 * 16 * N
     path_gbs = b_alg * value / 1e9
 
     # ---- plain (unfused torch) path on the same GPU
@@ -699,7 +702,7 @@ def main(argv=None) -> int:
         g = epoch_small["graph"]["ms_per_epoch"]
         cpu_small["gpu_graph_epoch_ms"] = g
         cpu_small["gpu_speedup_10_epochs"] = round(cpu_small["ten_epoch_wall_s"] * 1e3 / (10 * g), 1)
-    modalities = None if args.no_modalities else measure_modalities(dev, args)
+    modalities = None if args.no_modalities else measure_modalities(dev, args, copy_peak)
     serving = None if args.no_serve else measure_serving(dev, args)
 
     out = {
@@ -727,7 +730,10 @@ def main(argv=None) -> int:
                      "measured_copy_peak": copy_peak,
                      "frac_of_copy_peak": round(dom_gbs / copy_peak, 4) if copy_peak else None},
         "path_roofline": {"bytes_per_pattern": b_alg, "achieved": round(path_gbs, 1), "unit": "GB/s",
-                          "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
+                          "frac": round(path_gbs / HBM_PEAK_GBS, 4),
+                          "frac_of_copy_peak": round(path_gbs / copy_peak, 4) if copy_peak else None,
+                          "io_bytes_per_pattern": b_io,
+                          "io_frac": round(b_io * value / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
         "tuning": tuning,
         "plain_path": plain,
