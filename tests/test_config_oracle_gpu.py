@@ -1,6 +1,9 @@
 """FP64-oracle error of the fused HIP steps at each benchmark configuration's
 full size, on that configuration's own synthetic recipe:
 
+* conventional, CPU-runnable case: object 512^2, probe 128^2 (radius 32,
+  defocus), 15 x 15 raster (step 24, +-2 px jitter), Poisson 1e4; its first
+  mini-batch of 32 and all 225 positions in one call;
 * conventional: object 2048^2, probe 512^2 (radius 128, defocus), the first 64
   positions of the 30 x 30 raster (step 52, +-3 px jitter), Poisson 1e5;
 * Fourier: spectrum 1024^2, pupil 256^2 (radius 48, random defocus), 15 x 15
@@ -8,7 +11,10 @@ full size, on that configuration's own synthetic recipe:
 * near-field: object 1024^2, 512^2 diffuser probe, 64 positions within
   +-32 px, angular spectrum at Fresnel number 20, Poisson 1e4;
 * coded: 1024^2, 7 x 7 translations on an 8 px pitch, H1 = H2 = the same
-  angular spectrum, Poisson 1e4.
+  angular spectrum, Poisson 1e4;
+* the five request classes of the served mix (ptychox_amd.serving), each with
+  the data and initial guesses a request arrives with: the server's first
+  epoch is this fused step.
 
 Each step is evaluated at two estimates against the FP64 oracle's loss and
 gradients (oracle/ptycho_oracle.py, restating python/train_ptychography.py:
@@ -100,6 +106,29 @@ def _check(name, state, hip, t32, conditioned=True):
         strict = conditioned and state == "initial guess"
         bar = TOL if strict else max(TOL, 2.0 * t32[k])
         assert hip[k] < bar, (name, state, k, hip[k], t32[k])
+
+
+@pytest.mark.parametrize("B", [32, 225])
+def test_conventional_cpu_case_vs_fp64_oracle(gpu, B):
+    from ptychox_amd import ptychoX_base
+    from ptychox_amd.base import BaseStep
+    from ptychox_amd.synthetic import conventional_dataset
+
+    d = conventional_dataset(512, 128, 32.0, 15, 24, 24, 2, 1e4, 4.0, gpu, seed=0)
+    pos, y = d.positions[:B].contiguous(), d.measured[:B].contiguous()
+    step = BaseStep((512, 512), (128, 128), B, gpu, keep_amplitude=True)
+    states = {"initial guess": (_uniform_real((512, 512), 0, gpu), _aperture(d.probe)),
+              "near convergence": (_perturb(d.sample, 9), _perturb(d.probe, 10))}
+    for state, (s, p) in states.items():
+        gs, gp = torch.empty_like(s), torch.empty_like(p)
+        lb = step(s, p, pos, y, gs, gp)
+        torch.cuda.synchronize()
+        sn, pn, posn, yn = _np(s), _np(p), _np(pos), _np(y)
+        _, per, gs_ref, gp_ref = po.base_train_step(sn, pn, posn, yn)
+        ref = {"per": per, "g1": gs_ref, "g2": gp_ref, "amp": po.base_forward(sn, pn, posn)["amplitude"]}
+        hip = _errors(lb, step.amplitude, gs, gp, ref)
+        t32 = _errors(*_torch_fp32(lambda a, b: ptychoX_base(a, b, pos), [s, p], y), ref)
+        _check(f"conventional 512^2/128^2 B={B}", state, hip, t32)
 
 
 def test_conventional_full_size_vs_fp64_oracle(gpu):
@@ -200,3 +229,59 @@ def test_coded_full_size_vs_fp64_oracle(gpu):
         hip = _errors(lb, step.amplitude, go, gc, ref)
         t32 = _errors(*_torch_fp32(lambda a, b: ptychoX_coded(a, b, t, t, sh, 1), [o, c], y), ref)
         _check("coded 1024^2 B=49", state, hip, t32)
+
+
+# one request of each class of the served mix (its first epoch's fused step)
+SERVED = [("conventional", 128, 400), ("conventional", 256, 400), ("fourier", 128, 225),
+          ("nearfield", 512, 64), ("coded", 512, 25)]
+
+
+@pytest.mark.parametrize("kind,size,views", SERVED)
+def test_served_request_classes_vs_fp64_oracle(gpu, kind, size, views):
+    from ptychox_amd import ptychoX_base, ptychoX_coded, ptychoX_fourier, ptychoX_nearfield
+    from ptychox_amd import serving as sv
+    from ptychox_amd.base import BaseStep
+    from ptychox_amd.coded import CodedStep
+    from ptychox_amd.fpm import FpmStep
+    from ptychox_amd.nearfield import NearFieldStep
+
+    req = sv.Request(0.0, 7, kind, size, views, seed=7)
+    a = {k: v.to(gpu) for k, v in sv.make_problem(req).arrays.items()}
+    o, q, y = a["object"], a["aux"], a["measured"]
+    go, gq = torch.empty_like(o), torch.empty_like(q)
+    V = y.shape[0]
+    on, qn, yn = _np(o), _np(q), _np(y)
+    if kind == "conventional":
+        step = BaseStep(o.shape, q.shape, V, gpu, keep_amplitude=True)
+        lb = step(o, q, a["positions"], y, go, gq)
+        _, per, g1, g2 = po.base_train_step(on, qn, _np(a["positions"]), yn)
+        amp = po.base_forward(on, qn, _np(a["positions"]))["amplitude"]
+        op = lambda u, v: ptychoX_base(u, v, a["positions"])
+    elif kind == "fourier":
+        step = FpmStep(o.shape, q.shape, V, gpu, keep_amplitude=True)
+        lb = step(o, q, a["positions"], y, go, gq)
+        _, per, g1, g2 = po.fpm_train_step(on, qn, _np(a["positions"]), yn)
+        amp = po.fpm_forward(on, qn, _np(a["positions"]))["amplitude"]
+        op = lambda u, v: ptychoX_fourier(u, v, a["positions"])
+    elif kind == "nearfield":
+        step = NearFieldStep(o.shape, q.shape, V, gpu, keep_amplitude=True)
+        lb = step(o, q, a["transfer"], a["positions"], y, go, gq)
+        tn, posn = _np(a["transfer"]), _np(a["positions"])
+        _, per, g1, g2 = po.nearfield_train_step(on, qn, tn, posn, yn)
+        amp = po.nearfield_forward(on, qn, tn, posn)["amplitude"]
+        op = lambda u, v: ptychoX_nearfield(u, v, a["transfer"], a["positions"])
+    else:
+        step = CodedStep(o.shape, V, 1, gpu, keep_amplitude=True)
+        t, sh = a["transfer"], a["shifts"]
+        lb = step(o, q, t, t, sh, y, go, gq)
+        tn, shn = _np(t), _np(sh)
+        fwd = po.coded_forward(on, qn, tn, tn, shn, 1)
+        _, per, g = po.mse_amplitude_loss(fwd["amplitude"], yn)
+        g1, g2 = po.coded_backward(g, fwd, qn, tn, tn, 1)
+        amp = fwd["amplitude"]
+        op = lambda u, v: ptychoX_coded(u, v, t, t, sh, 1)
+    torch.cuda.synchronize()
+    ref = {"per": per, "g1": g1, "g2": g2, "amp": amp}
+    hip = _errors(lb, step.amplitude, go, gq, ref)
+    t32 = _errors(*_torch_fp32(op, [o, q], y), ref)
+    _check(f"served {kind} {size} x{V}", "initial guess", hip, t32)
