@@ -139,11 +139,12 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     }
     const auto rt = buffer_rsrc(tb + ti.frame * fsz, live ? fbytes : 0u);
     const auto rm = buffer_rsrc(measured + mframe * fsz, live ? fbytes / 2 : 0u);
+    // the frame loads first: the forward transform waits for them only, the
+    // measured amplitudes (needed after it) stay in flight meanwhile
 #pragma unroll
-    for (int q = 0; q < E; ++q) {
-      dv[q] = buf_load_f2(rt, lane * 8u, (q * rstep + col0) * 8u);
-      dm[q] = buf_load_f1(rm, lane * 4u, (q * rstep + col0) * 4u);
-    }
+    for (int q = 0; q < E; ++q) dv[q] = buf_load_f2(rt, lane * 8u, (q * rstep + col0) * 8u);
+#pragma unroll
+    for (int q = 0; q < E; ++q) dm[q] = buf_load_f1(rm, lane * 4u, (q * rstep + col0) * 4u);
   };
   auto process = [&](int64_t kk, float2 (&v)[E], float (&m)[E]) {
     const TileIdx ti = split_tile(xcd_paired_block((unsigned)kk, nt), tiles_per_frame);
