@@ -457,6 +457,14 @@ def measure_serving(dev, args) -> dict:
         return [sv.Request(float(t[i]), r.rid, r.kind, r.size, r.views, r.seed, r.epochs, r.lr)
                 for i, r in enumerate(base)]
 
+    # the FFT schedule tuned for the mix: one request of each class, their steps' sum
+    if args.no_tune:
+        serve_tuning = None
+    else:
+        res = sv.autotune_classes(base, probs, dev)
+        serve_tuning = {"table": res["table"], "step_sum_before_ms": round(res["before_ms"], 4),
+                        "step_sum_after_ms": round(res["after_ms"], 4)}
+        print(f"[bench] serving autotune: {serve_tuning}", file=sys.stderr, flush=True)
     sv.Server(dev).run(base[:10], probs)  # warm-up: first launches, graph machinery
     srv = sv.Server(dev)
     sat_done = srv.run(base, probs)
@@ -528,6 +536,7 @@ def measure_serving(dev, args) -> dict:
         "workload": "mixed stream: conventional 128/512 and 256/1024 (100-400 positions), Fourier 128^2/512^2 "
                     "(81|225 LEDs), near-field 512/1024 (64), coded 512^2 (25); 20 Adam epochs each",
         "requests": n,
+        "tuning": serve_tuning,
         "saturation": {"server": sat, "plain": plain_sat,
                        "speedup": round(sat["throughput_req_per_s"] / plain_rps, 1)},
         "poisson": rates,

@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(256)
   using Cf = RowCfg<N, E>;
   constexpr int T = Cf::T, S = Cf::S;
   __shared__ float2 lds[S * lds_len<N>()];
-  __shared__ float red[256];
+  __shared__ float red[4];  // per-wave loss sums
   const int seq = threadIdx.x / T;
   const int t = threadIdx.x % T;
   const int64_t row = (int64_t)blockIdx.x * S + seq;
@@ -105,12 +105,24 @@ __global__ void __launch_bounds__(256)
     if (valid && amp_out) amp_out[k] = a;
     v[q] = c_scale(e, gscale * d * ai.inv);
   }
-  red[threadIdx.x] = lsum;
-  __syncthreads();  // also orders the LDS reuse of the second transform
-  if (valid && t == 0) {
-    float sum = 0.0f;
-    for (int k = 0; k < T; ++k) sum += red[seq * T + k];  // fixed order
-    loss_part[row] = sum;
+  // row sum in a fixed order: butterfly within each wave, then the row's
+  // waves in order (a row is T >= 64 threads, whole waves; shorter rows share
+  // a wave as T-lane groups).  The barrier also orders the LDS reuse of the
+  // second transform.
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1)
+    if (off < T) lsum += __shfl_xor(lsum, off, 64);
+  if constexpr (T >= 64) {
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lsum;
+    __syncthreads();
+    if (valid && t == 0) {
+      float sum = 0.0f;
+      for (int k = 0; k < T / 64; ++k) sum += red[(seq * T >> 6) + k];
+      loss_part[row] = sum;
+    }
+  } else {
+    if (valid && t == 0) loss_part[row] = lsum;
+    __syncthreads();
   }
   fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
   if (valid) {

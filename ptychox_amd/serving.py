@@ -234,6 +234,50 @@ def footprint(req: Request) -> int:
     return sum(_rounded(t) for t in tensors)
 
 
+def autotune_classes(requests: list[Request], problems: dict[int, Problem], device="cuda", reps: int = 10,
+                     rounds: int = 2) -> dict:
+    """Tune the FFT schedule (:mod:`ptychox_amd.autotune`) for the request
+    classes in ``requests``: one representative per (kind, size) -- the one
+    with the most views -- and, as the objective, the time of one fused step
+    of each in turn, since the classes share table entries (a 512-point row
+    pass serves near-field and coded requests alike).  The tables are
+    process-wide: tune before serving, not while requests run (footprints and
+    workspaces are sized from the tables in force).  Returns the autotuner's
+    result (table, step-sum time before and after)."""
+    from . import autotune as at
+
+    dev = torch.device(device)
+    rep: dict = {}
+    for r in requests:
+        key = (r.kind, r.size)
+        if key not in rep or r.views > rep[key].views:
+            rep[key] = r
+    stream = torch.cuda.Stream(device=dev)
+    jobs = [Job(r, problems[r.rid], dev, stream, graph=False) for r in rep.values()]
+    keys: set = set()
+    for r in rep.values():
+        sh = shapes(r)
+        if r.kind in ("conventional", "nearfield"):
+            keys |= set(at.keys_for_scan(sh["aux"]))
+        elif r.kind == "fourier":
+            keys |= set(at.keys_for_fpm(sh["object"], sh["aux"]))
+        else:
+            keys |= set(at.keys_for_coded(sh["object"]))
+
+    def run():
+        with torch.cuda.stream(stream):
+            for j in jobs:
+                j._fused_step()
+        torch.cuda.current_stream(dev).wait_stream(stream)
+
+    try:
+        return at.autotune(run, sorted(keys), reps=reps, rounds=rounds)
+    finally:
+        torch.cuda.synchronize(dev)
+        for j in jobs:
+            j.release()
+
+
 # ----------------------------------------------------------------------------- device-resident job
 
 def hyper_table(lr: float, epochs: int) -> torch.Tensor:

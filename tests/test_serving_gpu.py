@@ -98,3 +98,33 @@ def test_finished_job_refuses_another_epoch(gpu, problems):
     torch.cuda.synchronize()
     assert int(job.counter) == req.epochs + 1
     assert torch.equal(job.obj, obj) and torch.equal(job.aux, aux)
+
+
+def _reset_tuning():
+    from ptychox_amd import _lib
+
+    for kind in range(6):
+        for n in (16, 32, 64, 128, 256, 512, 1024, 2048):
+            _lib.call("ptx_tune_set", kind, n, 0)
+            _lib.call("ptx_tune_set_groups", kind, n, 0)
+
+
+def test_schedule_tuned_for_the_mix_keeps_alone_equal_mix(gpu, problems):
+    # the mix's own schedule (one request per class, their steps' sum): every
+    # setting it leaves is a compiled variant, and under it each request is
+    # still bitwise the same alone and in the mix (footprints follow the table)
+    try:
+        res = sv.autotune_classes(MIX, problems, gpu, reps=3, rounds=1)
+        print("tuned for the mix:", res["table"], res["before_ms"], res["after_ms"], flush=True)
+        for key, value in res["table"].items():
+            assert value in ((0, 8, 16) if "@" not in key else range(0, 65537)), (key, value)
+        srv = sv.Server(gpu, hbm_budget=4e9, slots=4, keep_results=True)
+        mixed = srv.run(MIX, problems)
+        for req, m in zip(MIX, mixed):
+            a = _alone(gpu, req, problems[req.rid])
+            assert torch.equal(m.object, a.object), req
+            assert torch.equal(m.aux, a.aux), req
+            assert m.final_loss == a.final_loss
+            assert 0 < m.resident <= m.footprint, m
+    finally:
+        _reset_tuning()
