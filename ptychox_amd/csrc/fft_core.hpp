@@ -217,6 +217,28 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Ordering of LDS accesses among the lanes of one wave only: the LDS performs
+// a wave's operations in issue order, so a sequence held entirely by one wave
+// (a row of T <= 64 threads; rows never straddle waves since T divides 64)
+// needs no work-group barrier between an exchange's writes and reads -- only
+// that the compiler keeps them in program order.  The other waves of the
+// work-group then run their transforms without waiting for each other.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// The synchronisation an exchange needs: the wave's own (WAVE: the sequence
+// is held by one wave) or the work-group's.
+template <bool WAVE>
+__device__ __forceinline__ void seq_sync() {
+  if constexpr (WAVE)
+    wave_lds_sync();
+  else
+    lds_barrier();
+}
+
 // LDS_TW: the stage's base twiddles come from `lds_tw`, an LDS copy of
 // w_N^k (k < N, forward direction; see load_lds_twiddles) instead of the
 // global table.  REG_TW: every twiddle of the stage comes from `reg_tw`
@@ -224,7 +246,7 @@ __device__ __forceinline__ void lds_barrier() {
 // PAIR_TW (global-table twiddles): w and w^4 from one load of the pair table
 // rather than two loads of the single table (same values).
 template <int N, int E, int DIR, int R, int NS, bool LAST, bool SYNC_FIRST, bool LDS_TW = false, bool REG_TW = false,
-          bool PAIR_TW = true>
+          bool PAIR_TW = true, bool WAVE = false>
 __device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restrict__ lds,
                                           const float2* __restrict__ lds_tw = nullptr,
                                           const float2* reg_tw = nullptr) {
@@ -232,7 +254,7 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restr
   constexpr int NB = E / R;               // butterflies per thread
   constexpr int TWS = TW_MAX / (NS * R);  // twiddle stride into the table
   // single-buffer mode: everyone must have finished reading the previous exchange
-  if constexpr (SYNC_FIRST) lds_barrier();
+  if constexpr (SYNC_FIRST) seq_sync<WAVE>();
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     const int j = t + i * T;
@@ -295,7 +317,7 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, float2* __restr
     }
   }
   if constexpr (!LAST) {
-    lds_barrier();
+    seq_sync<WAVE>();
     // pad(t + q T) = pad(t) + pad(q T) likewise (t < T, T a power of two)
     const float2* rb = lds + lds_pad(t);
 #pragma unroll
@@ -317,7 +339,8 @@ __device__ __forceinline__ void load_lds_twiddles(float2* lds_tw) {
 
 // PINGPONG: alternate two LDS regions (one barrier per exchange); otherwise a
 // single region and two barriers per exchange (half the LDS footprint).
-template <int N, int E, int DIR, bool PINGPONG, int S, bool LDS_TW = false, bool PAIR_TW = true>
+// WAVE: the sequence is held by one wave (wave_lds_sync instead of barriers).
+template <int N, int E, int DIR, bool PINGPONG, int S, bool LDS_TW = false, bool PAIR_TW = true, bool WAVE = false>
 __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, float2* lds0, float2* lds1,
                                            const float2* lds_tw = nullptr) {
   using Sh = FftShape<N, E>;
@@ -326,9 +349,9 @@ __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, float2* lds0, 
     constexpr int NS = ipow(E, S);
     constexpr bool LAST = S == Sh::STAGES - 1;
     constexpr bool SYNC_FIRST = !PINGPONG && S > 0 && !LAST;
-    fft_stage<N, E, DIR, R, NS, LAST, SYNC_FIRST, LDS_TW, false, PAIR_TW>(v, t, (PINGPONG && (S & 1)) ? lds1 : lds0,
-                                                                          lds_tw);
-    fft_stages<N, E, DIR, PINGPONG, S + 1, LDS_TW, PAIR_TW>(v, t, lds0, lds1, lds_tw);
+    fft_stage<N, E, DIR, R, NS, LAST, SYNC_FIRST, LDS_TW, false, PAIR_TW, WAVE>(
+        v, t, (PINGPONG && (S & 1)) ? lds1 : lds0, lds_tw);
+    fft_stages<N, E, DIR, PINGPONG, S + 1, LDS_TW, PAIR_TW, WAVE>(v, t, lds0, lds1, lds_tw);
   }
 }
 
@@ -418,9 +441,23 @@ __device__ __forceinline__ void fft_seq(float2 (&v)[E], int t, float2* lds0, flo
 
 // The same transform with the stage twiddles read from an LDS table
 // (load_lds_twiddles<N>).
-template <int N, int E, int DIR>
+template <int N, int E, int DIR, bool WAVE = false>
 __device__ __forceinline__ void fft_seq_tw(float2 (&v)[E], int t, float2* lds, const float2* lds_tw) {
-  fft_stages<N, E, DIR, false, 0, true>(v, t, lds, nullptr, lds_tw);
+  fft_stages<N, E, DIR, false, 0, true, true, WAVE>(v, t, lds, nullptr, lds_tw);
+}
+
+// A row of a row pass: thread t < T = N/E of the row's T consecutive threads
+// (row s = threadIdx.x / T).  With T <= 64 the row lies within one wave and
+// its exchanges synchronise that wave only.
+template <int N>
+constexpr bool row_in_wave(int E) { return N / E <= 64; }
+template <int N, int E, int DIR>
+__device__ __forceinline__ void fft_row(float2 (&v)[E], int t, float2* lds) {
+  fft_stages<N, E, DIR, false, 0, false, true, row_in_wave<N>(E)>(v, t, lds, nullptr);
+}
+template <int N, int E, int DIR>
+__device__ __forceinline__ void fft_row_tw(float2 (&v)[E], int t, float2* lds, const float2* lds_tw) {
+  fft_stages<N, E, DIR, false, 0, true, true, row_in_wave<N>(E)>(v, t, lds, nullptr, lds_tw);
 }
 
 // Elements per thread used for a transform of length N.

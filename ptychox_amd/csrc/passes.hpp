@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(256) k_rows(Op op, int64_t nrows) {
   const bool valid = row < nrows;
   float2 v[E];
   op.template load<N, E>(row, valid, t, v);
-  fft_seq<N, E, DIR>(v, t, &lds[seq * lds_len<N>()]);
+  fft_row<N, E, DIR>(v, t, &lds[seq * lds_len<N>()]);
   op.template store<N, E>(row, valid, t, v);
 }
 

@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(256)
   float2 v[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = valid ? tb[row * N + t + q * T] : c_zero();
-  fft_seq<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
+  fft_row<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int i = t + q * T;
@@ -115,8 +115,8 @@ __global__ void __launch_bounds__(256)
     if (valid) xfwd[row * N + i] = x;
     v[q] = c_mul(x, mask[y * N + i]);
   }
-  __syncthreads();
-  fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
+  seq_sync<row_in_wave<N>(E)>();  // the second transform rewrites the row's LDS
+  fft_row<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
   if (valid) store_row_strip<N, E>(out + (row - y) * N, (uint32_t)H, (uint32_t)y, t, v);
 }
 
@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(256)
   float2 v[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = valid ? tb[row * N + t + q * T] : c_zero();
-  fft_seq<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
+  fft_row<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
   float lsum = 0.0f;
 #pragma unroll
   for (int q = 0; q < E; ++q) {
@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1)
     if (off < T) lsum += __shfl_xor(lsum, off, 64);
-  if constexpr (T >= 64) {
+  if constexpr (T > 64) {
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lsum;
     __syncthreads();
     if (valid && t == 0) {
@@ -264,10 +264,11 @@ __global__ void __launch_bounds__(256)
       loss_part[row] = sum;
     }
   } else {
+    // the row lies within one wave, whose lanes all hold its sum
     if (valid && t == 0) loss_part[row] = lsum;
-    __syncthreads();
+    seq_sync<true>();  // the second transform rewrites the row's LDS
   }
-  fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
+  fft_row<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
   if (valid) {
     const int64_t y = row % H;
     store_row_strip<N, E>(out + (row - y) * N, (uint32_t)H, (uint32_t)y, t, v);
@@ -328,7 +329,7 @@ __global__ void __launch_bounds__(256)
     int tt = t;
     asm volatile("" : "+v"(tt));  // keep the exchange addresses per view, not hoisted into registers
     float2* sl = &lds[seq * lds_len<N>()];
-    fft_seq_tw<N, E, +1>(v, tt, sl, s_tw);
+    fft_row_tw<N, E, +1>(v, tt, sl, s_tw);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const float2 gy = c_scale(v[q], inv_n);
@@ -336,10 +337,10 @@ __global__ void __launch_bounds__(256)
       v[q] = c_mulc(gy, cm[q]);
     }
     if (b + 1 < b1) fetch_x(b + 1, nx);
-    lds_barrier();
-    fft_seq_tw<N, E, -1>(v, tt, sl, s_tw);
+    seq_sync<row_in_wave<N>(E)>();  // the second transform rewrites the row's LDS
+    fft_row_tw<N, E, -1>(v, tt, sl, s_tw);
     if (yv) store_row_strip<N, E>(out + b * H * N, (uint32_t)H, (uint32_t)y, t, v);  // column-strip layout
-    lds_barrier();
+    seq_sync<row_in_wave<N>(E)>();  // next view's first LDS write vs this view's last read
   }
   if (yv) {
 #pragma unroll

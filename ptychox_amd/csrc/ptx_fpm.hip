@@ -110,10 +110,11 @@ __global__ void __launch_bounds__(256)
     float2* l1 = &lds1[sq * lds_len<N>()];
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = c_mul(v[q], pr[q]);
+    constexpr bool WAVE = row_in_wave<N>(E);  // each row within one wave
     if (second)
-      fft_stages<N, E, +1, true, 0, true>(v, tt, l1, l0, s_tw);
+      fft_stages<N, E, +1, true, 0, true, true, WAVE>(v, tt, l1, l0, s_tw);
     else
-      fft_stages<N, E, +1, true, 0, true>(v, tt, l0, l1, s_tw);
+      fft_stages<N, E, +1, true, 0, true, true, WAVE>(v, tt, l0, l1, s_tw);
     const auto ro = buffer_rsrc(out + (int64_t)b * h * N, frame_bytes);
 #pragma unroll
     for (int q = 0; q < E; ++q) buf_store_f2(v[q], ro, out_lane, (uint32_t)(q * T * 8));
@@ -273,7 +274,7 @@ __global__ void __launch_bounds__(256)
     const float2* src = tb + (b * h + r) * N;
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = rv ? src[t + q * T] : c_zero();
-    fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
+    fft_row<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
     if (rv) {
       const int64_t sy = (led[2 * b + 1] + r + H / 2) & (H - 1);
       const int64_t sx0 = led[2 * b] + W / 2;
@@ -292,7 +293,7 @@ __global__ void __launch_bounds__(256)
         }
       }
     }
-    __syncthreads();
+    seq_sync<row_in_wave<N>(E)>();  // next LED's first LDS write vs this LED's last LDS read
   }
   if (rv) {
     float2* d = gq_part + ((int64_t)ch * h + r) * N;

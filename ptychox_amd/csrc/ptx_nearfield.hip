@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(256)
   float2 v[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = valid ? p[t + q * T] : c_zero();
-  fft_seq<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
+  fft_row<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
   float lsum = 0.0f;
 #pragma unroll
   for (int q = 0; q < E; ++q) {
@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1)
     if (off < T) lsum += __shfl_xor(lsum, off, 64);
-  if constexpr (T >= 64) {
+  if constexpr (T > 64) {
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lsum;
     __syncthreads();
     if (valid && t == 0) {
@@ -121,10 +121,11 @@ __global__ void __launch_bounds__(256)
       loss_part[row] = sum;
     }
   } else {
+    // the row lies within one wave, whose lanes all hold its sum
     if (valid && t == 0) loss_part[row] = lsum;
-    __syncthreads();
+    seq_sync<true>();  // the second transform rewrites the row's LDS
   }
-  fft_seq<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
+  fft_row<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
   if (valid) {
 #pragma unroll
     for (int q = 0; q < E; ++q) p[t + q * T] = v[q];

@@ -134,10 +134,11 @@ __global__ void __launch_bounds__(256)
     float2* l1 = &lds1[sq * lds_len<N>()];
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = c_mul(v[q], pr[q]);
+    constexpr bool WAVE = row_in_wave<N>(E);  // each row within one wave
     if (second)
-      fft_stages<N, E, -1, true, 0, true>(v, tt, l1, l0, s_tw);
+      fft_stages<N, E, -1, true, 0, true, true, WAVE>(v, tt, l1, l0, s_tw);
     else
-      fft_stages<N, E, -1, true, 0, true>(v, tt, l0, l1, s_tw);
+      fft_stages<N, E, -1, true, 0, true, true, WAVE>(v, tt, l0, l1, s_tw);
     const auto ro = buffer_rsrc(out + (int64_t)b * h * N, frame_bytes);
 #pragma unroll
     for (int q = 0; q < E; ++q) buf_store_f2(v[q], ro, out_lane, (uint32_t)(q * T * 8));
@@ -193,7 +194,7 @@ __global__ void __launch_bounds__(256)
     const float2* src = tb + (b * h + r) * N;
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = rv ? src[t + q * T] : c_zero();
-    fft_seq<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
+    fft_row<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
     const int64_t x0 = pos[2 * b], y0 = pos[2 * b + 1];
     if (rv && window_ok(x0, y0, h, N, H, W)) {
       const int64_t obase = (y0 + r) * W + x0;
@@ -211,7 +212,7 @@ __global__ void __launch_bounds__(256)
         }
       }
     }
-    lds_barrier();  // next view's first LDS write vs this view's last LDS read
+    seq_sync<row_in_wave<N>(E)>();  // next view's first LDS write vs this view's last LDS read
   }
   if (rv) {
     float2* d = gp_part + ((int64_t)ch * h + r) * N;
