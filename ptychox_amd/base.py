@@ -210,7 +210,7 @@ class BaseStep:
         if B > self.batch:
             raise ValueError(f"batch {B} exceeds the planned batch {self.batch}")
         if loss_scale is None:
-            loss_scale = 1.0 / (B * self.h * self.w)
+            loss_scale = 1.0 / (B * self.h * self.w) if B > 0 else 0.0
         flags = self.flags | (_lib.PTX_FLAG_ACCUMULATE if accumulate else 0)
         if self._ws_bytes() > self.ws.numel():
             self.ws = _lib.workspace(self._ws_bytes(), self.device)

@@ -237,7 +237,7 @@ class CodedStep:
         stream = _lib.stream_of(self.device)
         if self.loss_name == "mse":
             if loss_scale is None:
-                loss_scale = 1.0 / (B * h * w)
+                loss_scale = 1.0 / (B * h * w) if B > 0 else 0.0
             _lib.call(
                 "ptx_coded_step",
                 _lib.ptr(object_field), self.H, self.W, _lib.ptr(coded_surface), _lib.ptr(transfer_1),

@@ -599,9 +599,10 @@ struct RowChunkQuery {
   static ptx_status run(int64_t H, int64_t B, Chunks* c) {
     c->row_groups = (int)((H + RowCfg<N, E>::S - 1) / RowCfg<N, E>::S);
     int64_t want = (tuned_groups(PASS_VIEW_ROWS, N, 2048) + c->row_groups - 1) / c->row_groups;
-    want = want < 1 ? 1 : (want > B ? B : want);
-    c->row_chunk = (int)((B + want - 1) / want);
-    c->row_nchunks = (int)((B + c->row_chunk - 1) / c->row_chunk);
+    // (an empty batch plans one chunk of one view: nothing is launched for it)
+    want = want < 1 ? 1 : (B > 0 && want > B ? B : want);
+    c->row_chunk = B > 0 ? (int)((B + want - 1) / want) : 1;
+    c->row_nchunks = B > 0 ? (int)((B + c->row_chunk - 1) / c->row_chunk) : 1;
     return PTX_OK;
   }
 };
@@ -614,9 +615,9 @@ struct ColChunkQuery {
     c->col_c = W >= ColCfg<N, E>::CMAX ? ColCfg<N, E>::CMAX : 8;
     c->col_tiles = (int)(W / c->col_c);
     int64_t want = (tuned_groups(PASS_COL_LOOP, N, 256) + c->col_tiles - 1) / c->col_tiles;
-    want = want < 1 ? 1 : (want > B ? B : want);
-    c->col_chunk = (int)((B + want - 1) / want);
-    c->col_nchunks = (int)((B + c->col_chunk - 1) / c->col_chunk);
+    want = want < 1 ? 1 : (B > 0 && want > B ? B : want);
+    c->col_chunk = B > 0 ? (int)((B + want - 1) / want) : 1;
+    c->col_nchunks = B > 0 ? (int)((B + c->col_chunk - 1) / c->col_chunk) : 1;
     return PTX_OK;
   }
 };

@@ -202,7 +202,7 @@ class NearFieldStep:
             if t.device != self.device or t.dtype != dt or not t.is_contiguous():
                 raise ValueError(f"{name} must be a contiguous {dt} tensor on {self.device}")
         if loss_scale is None:
-            loss_scale = 1.0 / (B * self.h * self.w)
+            loss_scale = 1.0 / (B * self.h * self.w) if B > 0 else 0.0
         flags = self.flags | (_lib.PTX_FLAG_ACCUMULATE if accumulate else 0)
         if self._ws_bytes() > self.ws.numel():
             self.ws = _lib.workspace(self._ws_bytes(), self.device)

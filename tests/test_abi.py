@@ -77,6 +77,17 @@ def test_stream_copy_rejects_bad_arguments_without_gpu(lib):
     assert st == 1 and b"overlap" in h.ptx_last_error()
 
 
+def test_workspace_queries_of_an_empty_batch(lib):
+    # no views: every query answers (a zero-view plan once divided by the view count)
+    for name, args in (("ptx_base_backward_workspace", (256, 256, 64, 64, 0, 0)),
+                       ("ptx_base_step_workspace", (256, 256, 64, 64, 0, 0)),
+                       ("ptx_fpm_step_workspace", (256, 256, 64, 64, 0, 0)),
+                       ("ptx_nearfield_step_workspace", (256, 256, 64, 64, 0, 0)),
+                       ("ptx_coded_workspace", (64, 64, 0, 1, 0)),
+                       ("ptx_coded_workspace", (64, 64, 0, 2, 1))):
+        assert lib.query(name, *args) >= 0, name
+
+
 def test_workspace_queries(lib):
     n = lib.query("ptx_base_backward_workspace", 2048, 2048, 512, 512, 64, 0)
     assert n >= 64 * 512 * 512 * 8
