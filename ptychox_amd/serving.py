@@ -312,11 +312,15 @@ _HYPER_CACHE: dict = {}
 
 def device_hyper_table(lr: float, epochs: int, device: torch.device) -> torch.Tensor:
     """``hyper_table(lr, epochs)`` on ``device``, built once per (lr, epochs,
-    device) and shared read-only by the requests that use it."""
+    device) and shared read-only by the requests that use it.  Requests read
+    it on their own streams, so the upload is complete before it is handed
+    out (a synchronisation of the uploading stream, once per table)."""
     key = (float(lr), int(epochs), str(device))
     t = _HYPER_CACHE.get(key)
     if t is None:
-        t = _HYPER_CACHE[key] = hyper_table(lr, epochs).to(device)
+        t = hyper_table(lr, epochs).to(device)
+        torch.cuda.current_stream(device).synchronize()
+        _HYPER_CACHE[key] = t
     return t
 
 
