@@ -265,6 +265,8 @@ def autotune_classes(requests: list[Request], problems: dict[int, Problem], devi
             keys |= set(at.keys_for_coded(sh["object"]))
 
     def run():
+        # bracketed by the timing events the autotuner records on the current stream
+        stream.wait_stream(torch.cuda.current_stream(dev))
         with torch.cuda.stream(stream):
             for j in jobs:
                 j._fused_step()
