@@ -101,19 +101,29 @@ __global__ void __launch_bounds__(256)
   __shared__ float2 lds[S * lds_len<N>()];
   const int seq = threadIdx.x / T;
   const int t = threadIdx.x % T;
-  const int64_t row = (int64_t)blockIdx.x * S + seq;
+  const int64_t r0 = (int64_t)blockIdx.x * S;
+  const int64_t row = r0 + seq;
   const bool valid = row < nrows;
-  const int64_t y = valid ? row % H : 0;
+  const int64_t y = row & (H - 1);  // H is a power of two
+  // the work-group's rows as one buffer of each array: rows past nrows lie
+  // beyond it, so their loads return zero and their stores are dropped -- the
+  // loads need no branches (which would make the compiler wait for each load
+  // before issuing the next)
+  const uint32_t live = (uint32_t)((nrows - r0 < S ? nrows - r0 : S) * N);
+  const uint32_t el = (uint32_t)(seq * N + t);
   float2 v[E];
+  {
+    const auto rt = buffer_rsrc(tb + r0 * N, live * 8u);
 #pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = valid ? tb[row * N + t + q * T] : c_zero();
+    for (int q = 0; q < E; ++q) v[q] = buf_load_f2(rt, el * 8u, (uint32_t)(q * T * 8));
+  }
   fft_row<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
+  const auto rx = buffer_rsrc(xfwd + r0 * N, live * 8u);
 #pragma unroll
   for (int q = 0; q < E; ++q) {
-    const int i = t + q * T;
     const float2 x = c_scale(v[q], inv_n);
-    if (valid) xfwd[row * N + i] = x;
-    v[q] = c_mul(x, mask[y * N + i]);
+    buf_store_f2(x, rx, el * 8u, (uint32_t)(q * T * 8));
+    v[q] = c_mul(x, mask[y * N + t + q * T]);
   }
   seq_sync<row_in_wave<N>(E)>();  // the second transform rewrites the row's LDS
   fft_row<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
@@ -192,7 +202,8 @@ struct CodedGradZRows {
   template <int N, int E>
   __device__ __forceinline__ void load(int64_t row, bool valid, int t, float2 (&v)[E]) const {
     constexpr int T = N / E;
-    const int64_t b = valid ? row / H : 0, y = valid ? row % H : 0;
+    // (H is a power of two)
+    const int64_t b = valid ? row >> __builtin_ctzll((uint64_t)H) : 0, y = valid ? row & (H - 1) : 0;
     const int64_t lw = N / ds;
     const float* srow = sc + (b * (H / ds) + y / ds) * lw;
 #pragma unroll
@@ -211,9 +222,8 @@ struct CodedGradZRows {
   }
   template <int N, int E>
   __device__ __forceinline__ void store(int64_t row, bool valid, int t, float2 (&v)[E]) const {
-    constexpr int T = N / E;
     if (!valid) return;
-    const int64_t y = row % H;
+    const int64_t y = row & (H - 1);  // H is a power of two
     float2* d = out + (row - y) * N;  // the frame, column-strip layout
     store_row_strip<N, E>(d, (uint32_t)H, (uint32_t)y, t, v);
   }
@@ -232,23 +242,41 @@ __global__ void __launch_bounds__(256)
   __shared__ float red[4];  // per-wave loss sums
   const int seq = threadIdx.x / T;
   const int t = threadIdx.x % T;
-  const int64_t row = (int64_t)blockIdx.x * S + seq;
+  const int64_t r0 = (int64_t)blockIdx.x * S;
+  const int64_t row = r0 + seq;
   const bool valid = row < nrows;
+  // the work-group's rows as one buffer of each array: rows past nrows lie
+  // beyond it, so their loads return zero and their stores are dropped -- no
+  // branches around the loads, and the measured amplitudes are requested with
+  // the frame row, their latency hidden behind the forward transform
+  const uint32_t live = (uint32_t)((nrows - r0 < S ? nrows - r0 : S) * N);
+  const uint32_t el = (uint32_t)(seq * N + t);
   float2 v[E];
+  float m[E];
+  {
+    const auto rt = buffer_rsrc(tb + r0 * N, live * 8u);
+    const auto rm = buffer_rsrc(measured + r0 * N, live * 4u);
 #pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = valid ? tb[row * N + t + q * T] : c_zero();
+    for (int q = 0; q < E; ++q) v[q] = buf_load_f2(rt, el * 8u, (uint32_t)(q * T * 8));
+#pragma unroll
+    for (int q = 0; q < E; ++q) m[q] = buf_load_f1(rm, el * 4u, (uint32_t)(q * T * 4));
+  }
   fft_row<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
   float lsum = 0.0f;
+  // (an invalid row transforms zeros: amplitude 0 against 0, no loss, no gradient)
 #pragma unroll
   for (int q = 0; q < E; ++q) {
-    const int64_t k = row * N + t + q * T;
     const float2 e = c_scale(v[q], inv_n);
     const AbsInv ai = c_abs_inv(e, 1.0f / kCodedAmpEps);
-    const float a = ai.a;
-    const float d = valid ? a - measured[k] : 0.0f;
+    const float d = ai.a - m[q];
     lsum += d * d;
-    if (valid && amp_out) amp_out[k] = a;
+    m[q] = ai.a;
     v[q] = c_scale(e, gscale * d * ai.inv);
+  }
+  if (amp_out) {
+    const auto ra = buffer_rsrc(amp_out + r0 * N, live * 4u);
+#pragma unroll
+    for (int q = 0; q < E; ++q) buf_store_f1(m[q], ra, el * 4u, (uint32_t)(q * T * 4));
   }
   // row sum: butterfly within each wave, then the row's waves in order (a row
   // is T >= 64 threads, whole waves; shorter rows share a wave, T-lane groups)
@@ -270,7 +298,7 @@ __global__ void __launch_bounds__(256)
   }
   fft_row<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
   if (valid) {
-    const int64_t y = row % H;
+    const int64_t y = row & (H - 1);  // H is a power of two
     store_row_strip<N, E>(out + (row - y) * N, (uint32_t)H, (uint32_t)y, t, v);
   }
 }

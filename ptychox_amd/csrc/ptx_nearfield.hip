@@ -86,24 +86,41 @@ __global__ void __launch_bounds__(256)
   __shared__ float red[4];  // per-wave loss sums
   const int seq = threadIdx.x / T;
   const int t = threadIdx.x % T;
-  const int64_t row = (int64_t)blockIdx.x * S + seq;
+  const int64_t r0 = (int64_t)blockIdx.x * S;
+  const int64_t row = r0 + seq;
   const bool valid = row < nrows;
-  float2* p = tb + row * N;
+  // the work-group's rows as one buffer of each array: rows past nrows lie
+  // beyond it, so their loads return zero and their stores are dropped -- no
+  // branches around the loads, and the measured amplitudes are requested with
+  // the frame row, their latency hidden behind the forward transform
+  const uint32_t live = (uint32_t)((nrows - r0 < S ? nrows - r0 : S) * N);
+  const uint32_t el = (uint32_t)(seq * N + t);
+  const auto rt = buffer_rsrc(tb + r0 * N, live * 8u);
   float2 v[E];
+  float m[E];
+  {
+    const auto rm = buffer_rsrc(measured + r0 * N, live * 4u);
 #pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = valid ? p[t + q * T] : c_zero();
+    for (int q = 0; q < E; ++q) v[q] = buf_load_f2(rt, el * 8u, (uint32_t)(q * T * 8));
+#pragma unroll
+    for (int q = 0; q < E; ++q) m[q] = buf_load_f1(rm, el * 4u, (uint32_t)(q * T * 4));
+  }
   fft_row<N, E, +1>(v, t, &lds[seq * lds_len<N>()]);
   float lsum = 0.0f;
+  // (an invalid row transforms zeros: amplitude 0 against 0, no loss, no gradient)
 #pragma unroll
   for (int q = 0; q < E; ++q) {
-    const int64_t k = row * N + t + q * T;
     const float2 e = c_scale(v[q], inv_n);
     const AbsInv ai = c_abs_inv(e, 1.0f / kFltEps);
-    const float a = ai.a;
-    const float d = valid ? a - measured[k] : 0.0f;
+    const float d = ai.a - m[q];
     lsum += d * d;
-    if (valid && amp_out) amp_out[k] = a;
+    m[q] = ai.a;
     v[q] = c_scale(e, gscale * d * ai.inv);
+  }
+  if (amp_out) {
+    const auto ra = buffer_rsrc(amp_out + r0 * N, live * 4u);
+#pragma unroll
+    for (int q = 0; q < E; ++q) buf_store_f1(m[q], ra, el * 4u, (uint32_t)(q * T * 4));
   }
   // row sum in a fixed order: butterfly within each wave, then the row's
   // waves in order (a row is T >= 64 threads, whole waves; shorter rows share
@@ -126,10 +143,8 @@ __global__ void __launch_bounds__(256)
     seq_sync<true>();  // the second transform rewrites the row's LDS
   }
   fft_row<N, E, -1>(v, t, &lds[seq * lds_len<N>()]);
-  if (valid) {
 #pragma unroll
-    for (int q = 0; q < E; ++q) p[t + q * T] = v[q];
-  }
+  for (int q = 0; q < E; ++q) buf_store_f2(v[q], rt, el * 8u, (uint32_t)(q * T * 8));
 }
 
 template <int N, int E>
