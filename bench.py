@@ -12,6 +12,8 @@ HBM before timing starts.
 Prints ONE JSON line (see the keys below).  Also reported beside the headline:
  * plain_path  -- the same step as unfused torch ops on the same GPU (the
                   operator with use_cuda_kernels=False + mse_loss + backward);
+ * dropin_path -- the same step through the drop-in autograd operator on the
+                  HIP library (use_cuda_kernels=True + mse_loss + backward);
  * roofline    -- the dominant kernel's algorithmic bytes / its average duration
                   (hipEvents recorded by the library around that kernel);
  * cpu_baseline-- the reference's torch CPU path, restated literally (its
@@ -684,6 +686,33 @@ def main(argv=None) -> int:
                  "what": "ptychoX_base(use_cuda_kernels=False) + F.mse_loss + backward: unfused torch/rocFFT ops"}
         print(f"[bench] plain torch path: {pms:.3f} ms/step ({plain['value']} patterns/s)", file=sys.stderr, flush=True)
 
+    # ---- the drop-in operator on the HIP path, called as a user of the reference
+    # calls it: ptychoX_base(..., use_cuda_kernels=True) + mse_loss + backward
+    # (autograd forward and backward launches, the far field kept between them,
+    # one 4-number device-to-host copy per call for the window check)
+    dropin = None
+    if not args.no_plain:
+        ds_ = sample.clone().requires_grad_(True)
+        dp_ = probe.clone().requires_grad_(True)
+        kk = [0]
+
+        def dropin_step():
+            pos, meas = batches[kk[0] % nb]
+            kk[0] += 1
+            ds_.grad = None
+            dp_.grad = None
+            amp = ptychoX_base(ds_, dp_, pos, use_cuda_kernels=True)
+            torch.nn.functional.mse_loss(amp, meas).backward()
+
+        dms = _timed(dropin_step, 20)
+        dv = BATCH / (dms / 1e3)
+        dropin = {"value": round(dv, 1), "unit": "patterns/s", "ms_per_step": round(dms, 4),
+                  "speedup_vs_plain": round(dv / plain["value"], 2) if plain else None,
+                  "what": "ptychoX_base(use_cuda_kernels=True) + F.mse_loss + backward: the drop-in "
+                          "autograd operator on the HIP library (forward and backward launched separately)"}
+        print(f"[bench] drop-in HIP operator: {dms:.3f} ms/step ({dropin['value']} patterns/s)", file=sys.stderr,
+              flush=True)
+
     # ---- CPU baseline: the reference torch path restated literally, host cores
     cpu = cpu_small = None
     if not args.no_cpu:
@@ -746,6 +775,7 @@ def main(argv=None) -> int:
         "kernels": kernels,
         "tuning": tuning,
         "plain_path": plain,
+        "dropin_path": dropin,
         "cpu_baseline": cpu,
         "cpu_baseline_small_workload": cpu_small,
         "epoch": epoch,

@@ -76,10 +76,8 @@ def check_windows(positions_xy: torch.Tensor, h: int, w: int, H: int, W: int) ->
         # contract applies (an out-of-range window gives a zero frame and no
         # gradient, without touching memory outside the sample)
         return
-    ext = torch.stack(
-        [positions_xy[:, 0].min(), positions_xy[:, 1].min(), positions_xy[:, 0].max(), positions_xy[:, 1].max()]
-    ).tolist()
-    xmin, ymin, xmax, ymax = ext
+    lo, hi = torch.aminmax(positions_xy, dim=0)  # one reduction for both columns
+    (xmin, ymin), (xmax, ymax) = torch.stack((lo, hi)).tolist()
     if xmin < 0 or ymin < 0 or xmax + w > W or ymax + h > H:
         raise ValueError(
             f"scan window out of range: positions x in [{xmin}, {xmax}], y in [{ymin}, {ymax}] with a "
