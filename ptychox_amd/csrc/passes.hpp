@@ -234,18 +234,24 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   const uint32_t lg = strip_lg(width);
   const uint32_t sbase = strip_off(0u, col0, (uint32_t)N, lg);
   const uint32_t slane = strip_off((uint32_t)t, col0 + (uint32_t)c, (uint32_t)N, lg) - sbase;
-  auto fetch = [&](unsigned f, float2 (&dv)[E]) {
+  // rows q in [q0, q1) of frame f
+  auto fetch = [&](unsigned f, float2 (&dv)[E], int q0, int q1) {
     const bool live = f < nf;
     const auto rt = buffer_rsrc((STRIP ? src : tb) + (int64_t)(live ? f : 0u) * fsz, live ? fbytes : 0u);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
+      if (q < q0 || q >= q1) continue;
       if constexpr (STRIP)
         dv[q] = buf_load_f2(rt, slane * 8u, (sbase + ((uint32_t)(q * T) << lg)) * 8u);
       else
         dv[q] = buf_load_f2(rt, lane * 8u, (q * rstep + col0) * 8u);
     }
   };
-  auto process = [&](unsigned f, float2 (&v)[E]) {
+  // frame f from v; the next frame's loads are issued in two halves, its
+  // first E/2 rows before this call and the rest after this frame's forward
+  // transform, so that waves do not stall on a full memory address queue at
+  // the start of the transform
+  auto process = [&](unsigned f, float2 (&v)[E], unsigned fnext, float2 (&vnext)[E]) {
     // the exchange addresses depend only on (t, c); recompute them per frame
     // from the thread index (one register kept live across the loop) instead
     // of letting them be hoisted out of the loop and held in registers
@@ -255,6 +261,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     float2* l0 = &lds0[cc * col_stride<N, C>()];
     float2* l1 = &lds1[cc * col_stride<N, C>()];
     tw.template stages<-1>(v, tt, l0, l1);
+    fetch(fnext, vnext, E / 2, E);
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = CONJ ? c_mulc(v[q], tr[q]) : c_mul(v[q], tr[q]);
     if constexpr (Sh::EXCHANGES % 2 == 1)
@@ -267,17 +274,17 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   };
   float2 va[E], vb[E];
   unsigned f = group;
-  fetch(f, va);
-  fetch(f + ng, vb);  // in flight during this frame's transforms
-  process(f, va);
+  fetch(f, va, 0, E);
+  fetch(f + ng, vb, 0, E / 2);  // in flight during this frame's transforms
+  process(f, va, f + ng, vb);
   f += ng;
   while (f < nf) {
-    fetch(f + ng, va);
-    process(f, vb);
+    fetch(f + ng, va, 0, E / 2);
+    process(f, vb, f + ng, va);
     f += ng;
     if (f >= nf) break;
-    fetch(f + ng, vb);
-    process(f, va);
+    fetch(f + ng, vb, 0, E / 2);
+    process(f, va, f + ng, vb);
     f += ng;
   }
 }

@@ -119,7 +119,8 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   const unsigned nt = (unsigned)ntiles;
   LoopTwiddles<N, E> tw;  // thread t holds the same rows of every tile
   tw.init(t, s_tw);
-  // loads of tile kk into (dv, dm); the measured frame of a view is the
+  // Loads of tile kk: the frame rows q in [q0, q1) into dv (fetch_t), the
+  // measured amplitudes into dm (fetch_m); the measured frame of a view is the
   // batch's own (order == null) or measured[order[frame]].  A tile past the
   // end gets zero-length buffers: its loads are issued but return zero
   // without touching memory, so the loop's memory instructions are the same
@@ -127,7 +128,16 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   // stay in flight instead of draining it at every tile.
   // (tile and frame indices are compared as 32-bit scalars: a 64-bit ordered
   // compare needs vector registers, whose reuse would wait on in-flight stores)
-  auto fetch = [&](int64_t kk, float2 (&dv)[E], float (&dm)[E]) {
+  auto fetch_t = [&](int64_t kk, float2 (&dv)[E], int q0, int q1) {
+    const bool live = (unsigned)kk < nt;
+    const TileIdx ti = split_tile(xcd_paired_block(live ? (unsigned)kk : 0u, nt), tiles_per_frame);
+    const uint32_t col0 = (uint32_t)(ti.tile * C);
+    const auto rt = buffer_rsrc(tb + ti.frame * fsz, live ? fbytes : 0u);
+#pragma unroll
+    for (int q = 0; q < E; ++q)
+      if (q >= q0 && q < q1) dv[q] = buf_load_f2(rt, lane * 8u, (q * rstep + col0) * 8u);
+  };
+  auto fetch_m = [&](int64_t kk, float (&dm)[E]) {
     const bool live = (unsigned)kk < nt;
     const TileIdx ti = split_tile(xcd_paired_block(live ? (unsigned)kk : 0u, nt), tiles_per_frame);
     const uint32_t col0 = (uint32_t)(ti.tile * C);
@@ -137,16 +147,18 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
       // out of range: frame 0 (the positions of such a view are entry 0's too)
       mframe = ((o >> 32) == 0 && (uint32_t)o < (uint32_t)count) ? (int64_t)o : 0;
     }
-    const auto rt = buffer_rsrc(tb + ti.frame * fsz, live ? fbytes : 0u);
     const auto rm = buffer_rsrc(measured + mframe * fsz, live ? fbytes / 2 : 0u);
-    // the frame loads first: the forward transform waits for them only, the
-    // measured amplitudes (needed after it) stay in flight meanwhile
-#pragma unroll
-    for (int q = 0; q < E; ++q) dv[q] = buf_load_f2(rt, lane * 8u, (q * rstep + col0) * 8u);
 #pragma unroll
     for (int q = 0; q < E; ++q) dm[q] = buf_load_f1(rm, lane * 4u, (q * rstep + col0) * 4u);
   };
-  auto process = [&](int64_t kk, float2 (&v)[E], float (&m)[E]) {
+  // Tile kk from (v, m); the loads of the next tile knext are issued in two
+  // halves: its first E/2 frame rows before this call, the other frame rows
+  // and its measured amplitudes after this tile's forward transform.  Issued
+  // all at once, a tile's 2E loads filled the address queue of the memory
+  // pipeline, and waves stalled at their issue instead of starting the
+  // transform (column pass 69-71 -> 64-65 us at 2048^2/512^2/64 when split).
+  auto process = [&](int64_t kk, float2 (&v)[E], float (&m)[E], int64_t knext, float2 (&vnext)[E],
+                     float (&mnext)[E]) {
     const TileIdx ti = split_tile(xcd_paired_block((unsigned)kk, nt), tiles_per_frame);
     const uint32_t col0 = (uint32_t)(ti.tile * C);
     // the exchange addresses depend only on (t, c); recompute them per tile
@@ -157,6 +169,8 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     float2* l1 = &lds1[cc * col_stride<N, C>()];
     tw.template stages<-1>(v, tt, l0, l1);
     if constexpr (Sh::EXCHANGES % 2 == 1) lds_barrier();
+    fetch_t(knext, vnext, E / 2, E);
+    fetch_m(knext, mnext);
     float lsum = 0.0f;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -191,17 +205,18 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   float2 va[E], vb[E];
   float ma[E], mb[E];
   int64_t k = blockIdx.x;  // < ntiles: the grid never exceeds the tile count
-  fetch(k, va, ma);
-  fetch(k + gridDim.x, vb, mb);  // in flight during this tile's transforms
-  process(k, va, ma);
+  fetch_t(k, va, 0, E);
+  fetch_m(k, ma);
+  fetch_t(k + gridDim.x, vb, 0, E / 2);  // the next tile's first half, in flight during this tile
+  process(k, va, ma, k + gridDim.x, vb, mb);
   k += gridDim.x;
   while ((unsigned)k < nt) {
-    fetch(k + gridDim.x, va, ma);
-    process(k, vb, mb);
+    fetch_t(k + gridDim.x, va, 0, E / 2);
+    process(k, vb, mb, k + gridDim.x, va, ma);
     k += gridDim.x;
     if ((unsigned)k >= nt) break;
-    fetch(k + gridDim.x, vb, mb);
-    process(k, va, ma);
+    fetch_t(k + gridDim.x, vb, 0, E / 2);
+    process(k, va, ma, k + gridDim.x, vb, mb);
     k += gridDim.x;
   }
 }
