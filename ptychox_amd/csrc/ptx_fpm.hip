@@ -212,22 +212,39 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   const int64_t tile = ti.tile;
   const int64_t col = tile * C + c;
   const int64_t off = frame * N * width;  // frame base (uniform)
+  // the frame's buffers: one 32-bit offset per lane, the row steps as scalar
+  // offsets; the measured amplitudes are requested with the frame column, so
+  // their latency is hidden behind the forward transform
+  const uint32_t fbytes = (uint32_t)(N * width * 8);
+  const uint32_t lane = (uint32_t)(t * width + col);
+  const uint32_t rstep = (uint32_t)(T * width);
+  const auto rt = buffer_rsrc(tb + off, fbytes);
   float2 v[E];
+  float m[E];
+  {
+    const auto rm = buffer_rsrc(measured + off, fbytes / 2);
 #pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = tb[off + ((t + q * T) * (int)width + (int)col)];
+    for (int q = 0; q < E; ++q) v[q] = buf_load_f2(rt, lane * 8u, (uint32_t)q * rstep * 8u);
+#pragma unroll
+    for (int q = 0; q < E; ++q) m[q] = buf_load_f1(rm, lane * 4u, (uint32_t)q * rstep * 4u);
+  }
   // twiddles by separate table loads: measured faster here than the pair table (52 vs 60 us at 256 points)
   fft_seq<N, E, +1, false, false>(v, t, &lds[c * col_stride<N, C>()]);
   float lsum = 0.0f;
 #pragma unroll
   for (int q = 0; q < E; ++q) {
-    const int64_t k = off + ((t + q * T) * (int)width + (int)col);
     // a = inv_n |v|; inv_n / max(a, eps) = 1 / max(|v|, eps / inv_n)
     const AbsInv ai = c_abs_inv(v[q], inv_n / kFltEps);
     const float a = ai.a * inv_n;
-    const float d = a - measured[k];
+    const float d = a - m[q];
     lsum += d * d;
-    if (amp_out) amp_out[k] = a;
+    m[q] = a;
     v[q] = c_scale(v[q], gscale * d * ai.inv);
+  }
+  if (amp_out) {
+    const auto ra = buffer_rsrc(amp_out + off, fbytes / 2);
+#pragma unroll
+    for (int q = 0; q < E; ++q) buf_store_f1(m[q], ra, lane * 4u, (uint32_t)q * rstep * 4u);
   }
   // one loss partial per wave (no work-group barrier)
 #pragma unroll
@@ -237,7 +254,7 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
   lds_barrier();  // the second transform rewrites the LDS
   fft_seq<N, E, -1, false, false>(v, t, &lds[c * col_stride<N, C>()]);
 #pragma unroll
-  for (int q = 0; q < E; ++q) tb[off + ((t + q * T) * (int)width + (int)col)] = v[q];
+  for (int q = 0; q < E; ++q) buf_store_f2(v[q], rt, lane * 8u, (uint32_t)q * rstep * 8u);
 }
 
 // ---------------------------------------------------------------- R2: row FFT, pupil partials, spectrum scatter
