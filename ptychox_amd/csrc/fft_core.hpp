@@ -340,18 +340,21 @@ __device__ __forceinline__ void load_lds_twiddles(float2* lds_tw) {
 // PINGPONG: alternate two LDS regions (one barrier per exchange); otherwise a
 // single region and two barriers per exchange (half the LDS footprint).
 // WAVE: the sequence is held by one wave (wave_lds_sync instead of barriers).
-template <int N, int E, int DIR, bool PINGPONG, int S, bool LDS_TW = false, bool PAIR_TW = true, bool WAVE = false>
+// Stages S .. SEND - 1 only: a transform can be run in parts, with other work
+// (the next tile's loads) issued between its stages.
+template <int N, int E, int DIR, bool PINGPONG, int S, bool LDS_TW = false, bool PAIR_TW = true, bool WAVE = false,
+          int SEND = 64>
 __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, float2* lds0, float2* lds1,
                                            const float2* lds_tw = nullptr) {
   using Sh = FftShape<N, E>;
-  if constexpr (S < Sh::STAGES) {
+  if constexpr (S < Sh::STAGES && S < SEND) {
     constexpr int R = S < Sh::FULL ? E : Sh::REM;
     constexpr int NS = ipow(E, S);
     constexpr bool LAST = S == Sh::STAGES - 1;
     constexpr bool SYNC_FIRST = !PINGPONG && S > 0 && !LAST;
     fft_stage<N, E, DIR, R, NS, LAST, SYNC_FIRST, LDS_TW, false, PAIR_TW, WAVE>(
         v, t, (PINGPONG && (S & 1)) ? lds1 : lds0, lds_tw);
-    fft_stages<N, E, DIR, PINGPONG, S + 1, LDS_TW, PAIR_TW, WAVE>(v, t, lds0, lds1, lds_tw);
+    fft_stages<N, E, DIR, PINGPONG, S + 1, LDS_TW, PAIR_TW, WAVE, SEND>(v, t, lds0, lds1, lds_tw);
   }
 }
 
@@ -386,18 +389,18 @@ struct RegTwiddles {
   }
 };
 
-template <int N, int E, int DIR, bool PINGPONG, int S = 0>
+template <int N, int E, int DIR, bool PINGPONG, int S = 0, int SEND = 64>
 __device__ __forceinline__ void fft_stages_reg(float2 (&v)[E], int t, float2* lds0, float2* lds1,
                                                const RegTwiddles<N, E>& tw) {
   using Sh = FftShape<N, E>;
-  if constexpr (S < Sh::STAGES) {
+  if constexpr (S < Sh::STAGES && S < SEND) {
     constexpr int R = S < Sh::FULL ? E : Sh::REM;
     constexpr int NS = ipow(E, S);
     constexpr bool LAST = S == Sh::STAGES - 1;
     constexpr bool SYNC_FIRST = !PINGPONG && S > 0 && !LAST;
     fft_stage<N, E, DIR, R, NS, LAST, SYNC_FIRST, false, true>(v, t, (PINGPONG && (S & 1)) ? lds1 : lds0, nullptr,
                                                                tw.w[S > 0 ? S - 1 : 0]);
-    fft_stages_reg<N, E, DIR, PINGPONG, S + 1>(v, t, lds0, lds1, tw);
+    fft_stages_reg<N, E, DIR, PINGPONG, S + 1, SEND>(v, t, lds0, lds1, tw);
   }
 }
 
@@ -421,12 +424,13 @@ struct LoopTwiddles {
       table = s_tw;
     }
   }
-  template <int DIR, bool PINGPONG = true>
+  // stages S0 .. S1 - 1 of the transform (all by default)
+  template <int DIR, bool PINGPONG = true, int S0 = 0, int S1 = 64>
   __device__ __forceinline__ void stages(float2 (&v)[E], int t, float2* lds0, float2* lds1) const {
     if constexpr (REG)
-      fft_stages_reg<N, E, DIR, PINGPONG>(v, t, lds0, lds1, reg);
+      fft_stages_reg<N, E, DIR, PINGPONG, S0, S1>(v, t, lds0, lds1, reg);
     else
-      fft_stages<N, E, DIR, PINGPONG, 0, true>(v, t, lds0, lds1, table);
+      fft_stages<N, E, DIR, PINGPONG, S0, true, true, false, S1>(v, t, lds0, lds1, table);
   }
 };
 

@@ -151,12 +151,13 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
 #pragma unroll
     for (int q = 0; q < E; ++q) dm[q] = buf_load_f1(rm, lane * 4u, (q * rstep + col0) * 4u);
   };
-  // Tile kk from (v, m); the loads of the next tile knext are issued in two
-  // halves: its first E/2 frame rows before this call, the other frame rows
-  // and its measured amplitudes after this tile's forward transform.  Issued
-  // all at once, a tile's 2E loads filled the address queue of the memory
-  // pipeline, and waves stalled at their issue instead of starting the
-  // transform (column pass 69-71 -> 64-65 us at 2048^2/512^2/64 when split).
+  // Tile kk from (v, m); the loads of the next tile knext are spread over
+  // this tile's forward transform: its first E/2 frame rows before this call,
+  // the other frame rows after the transform's first stage, its measured
+  // amplitudes after the transform.  Issued all at once, a tile's 2E loads
+  // filled the address queue of the memory pipeline, and waves stalled at
+  // their issue instead of starting the transform (column pass 69-71 -> 62
+  // us at 2048^2/512^2/64 when spread this way).
   auto process = [&](int64_t kk, float2 (&v)[E], float (&m)[E], int64_t knext, float2 (&vnext)[E],
                      float (&mnext)[E]) {
     const TileIdx ti = split_tile(xcd_paired_block((unsigned)kk, nt), tiles_per_frame);
@@ -167,9 +168,10 @@ __global__ void __launch_bounds__((ColCfg<N, E>::T * C))
     asm volatile("" : "+v"(tt), "+v"(cc));
     float2* l0 = &lds0[cc * col_stride<N, C>()];
     float2* l1 = &lds1[cc * col_stride<N, C>()];
-    tw.template stages<-1>(v, tt, l0, l1);
-    if constexpr (Sh::EXCHANGES % 2 == 1) lds_barrier();
+    tw.template stages<-1, true, 0, 1>(v, tt, l0, l1);
     fetch_t(knext, vnext, E / 2, E);
+    tw.template stages<-1, true, 1>(v, tt, l0, l1);
+    if constexpr (Sh::EXCHANGES % 2 == 1) lds_barrier();
     fetch_m(knext, mnext);
     float lsum = 0.0f;
 #pragma unroll
