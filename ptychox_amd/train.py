@@ -17,13 +17,15 @@ mini-batch's LEDs / positions / shifts and measurements; the coded one adds
 the fd losses and the real non-negative coded-surface constraint.  Everything the graph reads
 that changes between epochs -- the permutation and the Adam step sizes -- lives
 in device buffers rewritten before each replay; per-position losses stay on the
-device and are reduced once per epoch.
+device and are reduced once per epoch.  Nothing in an epoch waits for the host:
+the step sizes upload from alternating page-locked buffers, and the epoch loss
+is reduced and copied behind the epoch and read only when asked for, so epochs
+are launched back to back.
 """
 
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass
 
 import torch
 
@@ -74,11 +76,32 @@ class AdamState:
         )
 
 
-@dataclass
 class EpochResult:
-    epoch: int
-    loss: float
-    lrs: tuple[float, float]
+    """One epoch: its number, the learning rates of the two parameter groups
+    and the drivers' epoch loss.  The loss of an epoch run on the device is
+    reduced behind the epoch and copied to page-locked host memory; reading
+    ``loss`` waits for that copy only (the first time), so the loop itself never
+    synchronises with the device."""
+
+    __slots__ = ("epoch", "lrs", "_value", "_host", "_event", "_count")
+
+    def __init__(self, epoch: int, loss: float | None, lrs: tuple[float, float], *, host=None, event=None,
+                 count: int = 1):
+        self.epoch = int(epoch)
+        self.lrs = lrs
+        self._value = None if loss is None else float(loss)
+        self._host, self._event, self._count = host, event, count
+
+    @property
+    def loss(self) -> float:
+        if self._value is None:
+            self._event.synchronize()
+            self._value = float(self._host) / self._count
+            self._host = self._event = None
+        return self._value
+
+    def __repr__(self) -> str:
+        return f"EpochResult(epoch={self.epoch}, loss={self.loss!r}, lrs={self.lrs!r})"
 
 
 class _EpochLoop:
@@ -118,7 +141,11 @@ class _EpochLoop:
         self.order = torch.arange(self.P, device=dev, dtype=torch.int64)
         self.loss = torch.zeros(self.P, dtype=torch.float32, device=dev)
         self.hyper = torch.zeros((2, self.nb, 4), dtype=torch.float32, device=dev)
-        self.hyper_host = torch.zeros((2, self.nb, 4), dtype=torch.float32, pin_memory=True)
+        # two page-locked staging buffers, alternating by epoch: a buffer is
+        # rewritten only once its previous upload (two epochs back) has finished
+        self.hyper_host = [torch.zeros((2, self.nb, 4), dtype=torch.float32, pin_memory=True) for _ in range(2)]
+        self._hyper_done: list[torch.cuda.Event | None] = [None, None]
+        self._prepared = 0
 
     def _batch(self, k: int, start: int, B: int) -> None:
         """Launch mini-batch k (views order[start:start+B]) on the current
@@ -144,14 +171,27 @@ class _EpochLoop:
         lrs = tuple(cosine_lr(b, self.min_lr, self.epochs, epoch, scale) for b in self.base_lrs)
         rows = [[list(adam_hyper(lr, self.betas[0], self.betas[1], self.eps, self.steps_done + k + 1)) + [0.0]
                  for k in range(self.nb)] for lr in lrs]
-        torch.cuda.current_stream(self.device).synchronize()  # the previous copy out of the pinned buffer is done
-        self.hyper_host.copy_(torch.tensor(rows, dtype=torch.float32))
-        self.hyper.copy_(self.hyper_host, non_blocking=True)
+        j = self._prepared % 2
+        self._prepared += 1
+        if self._hyper_done[j] is not None:
+            self._hyper_done[j].synchronize()  # this buffer's upload two epochs ago has finished
+        stream = torch.cuda.current_stream(self.device)
+        self.hyper_host[j].copy_(torch.tensor(rows, dtype=torch.float32))
+        self.hyper.copy_(self.hyper_host[j], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        self._hyper_done[j] = ev
         return lrs
 
-    def _epoch_loss(self) -> float:
-        # the drivers' epoch_loss: sum over mini-batches of loss * batch, / views
-        return float(self.loss.sum()) / self.P  # one sync per epoch
+    def _epoch_result(self, epoch: int, lrs) -> EpochResult:
+        # the drivers' epoch_loss: sum over mini-batches of loss * batch, / views,
+        # reduced on the device behind the epoch and read when asked for
+        stream = torch.cuda.current_stream(self.device)
+        host = torch.empty((), dtype=torch.float32, pin_memory=True)
+        host.copy_(self.loss.sum(), non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        return EpochResult(epoch, None, lrs, host=host, event=ev, count=self.P)
 
     def run_epoch(self, epoch: int, scale: float = 1.0) -> EpochResult:
         lrs = self._prepare(epoch, scale)
@@ -162,7 +202,7 @@ class _EpochLoop:
                 self._capture()
             self.graph.replay()
         self.steps_done += self.nb
-        return EpochResult(epoch, self._epoch_loss(), lrs)
+        return self._epoch_result(epoch, lrs)
 
     def _state(self):
         return (self.obj, self.aux, self.opt_o.m, self.opt_o.v, self.opt_a.m, self.opt_a.v)
@@ -359,17 +399,18 @@ class CodedReconstruction(_GatheredReconstruction):
         self.step(self.obj, self.aux, self.t1, self.t2, sh, meas, self.g_obj, self.g_aux, batch=B)
         self.loss[start:start + B].copy_(self.step.loss[:B])
 
-    def _epoch_loss(self) -> float:
+    def _epoch_result(self, epoch: int, lrs) -> EpochResult:
         if self.loss_name == "mse":
-            return super()._epoch_loss()
+            return super()._epoch_result(epoch, lrs)
         # fd losses are sums over a batch: epoch_loss = sum_k B_k * (sum over batch k) / views
+        # (read at once: this path synchronises once per epoch)
         per = self.loss.cpu()
         tot = 0.0
         for k in range(self.nb):
             s0 = k * self.batch
             Bk = min(self.batch, self.P - s0)
             tot += Bk * float(per[s0:s0 + Bk].sum())
-        return tot / self.P
+        return EpochResult(epoch, tot / self.P, lrs)
 
 
 def ctypes_offset(t: torch.Tensor, index: int):
