@@ -151,6 +151,33 @@ def test_graph_replays_match_eager(gpu):
         assert torch.equal(se, sg) and torch.equal(pe, pg)
 
 
+def test_epoch_losses_read_late_equal_losses_read_each_epoch(gpu):
+    # The loop never waits for the device between epochs: each epoch's step
+    # sizes upload from one of two alternating page-locked buffers, and its loss
+    # is reduced and copied behind it, read when asked for.  Epochs launched back
+    # to back with every loss read at the end must give bitwise the losses and
+    # unknowns of a run that reads each loss right after its epoch.
+    from ptychox_amd.train import ConventionalReconstruction
+
+    init_s, init_p, pos, meas = _problem(gpu, seed=2)
+    out = []
+    for read_each in (True, False):
+        gen = torch.Generator(device=gpu).manual_seed(11)
+        rec = ConventionalReconstruction(init_s, init_p, pos, meas, 8, lr=2e-2, epochs=7, generator=gen, graph=True)
+        results = []
+        for e in range(1, 8):
+            r = rec.run_epoch(e)
+            if read_each:
+                _ = r.loss
+            results.append(r)
+        out.append(([r.loss for r in results], rec.sample.clone(), rec.probe.clone()))
+    (l1, s1, p1), (l2, s2, p2) = out
+    print(f"[epoch losses] read each epoch {l1}", flush=True)
+    assert l1 == l2
+    assert torch.equal(s1, s2) and torch.equal(p1, p2)
+    assert l1[-1] < l1[0]
+
+
 def test_cpu_workload_recipe_trains_like_the_reference_loop(gpu):
     # the reference's CPU-runnable conventional case at its own sizes: object 512^2
     # (smooth random amplitude/phase, sigma 4), probe 128^2 (disc r = 32, defocus
