@@ -117,7 +117,7 @@ ptx_status ptx_stream_copy(const void* src, void* dst, size_t bytes, void* strea
  * loop over views, 4 row passes that loop over views keeping several frames'
  * values per thread (coded mask gradient), 5 row passes that evaluate the
  * amplitude loss between two transforms (coded step).  elems: 8 or 16, 0
- * restores the built-in default.
+ * restores the built-in default (8 above 16 points).
  * Process-wide; set it before launching work (the autotuner in
  * ptychox_amd/autotune.py measures and fills it). */
 ptx_status ptx_tune_set(int kind, int64_t n, int elems);
@@ -125,7 +125,7 @@ int ptx_tune_get(int kind, int64_t n);
 
 /* Work-group target of the passes that walk a chunk of views per work-group,
  * per pass kind and transform length: kind 0 the forward gathers (default
- * 1024), 1 the sorted-segment scatters (1024), 2 the transfer-function
+ * 1024), 1 the sorted-segment scatters (512), 2 the transfer-function
  * column passes (one work-group per CU) and the coded spectrum column pass
  * (256), 3 the coded spectrum-gradient column pass (256), 4 the coded
  * mask-gradient row pass (2048); kind 5 has no view chunks (the setting is

@@ -30,7 +30,7 @@ KIND_NAMES = {PASS_ROW: "row", PASS_ROW_LOOP: "row_loop", PASS_COL: "col", PASS_
               PASS_VIEW_ROWS: "view_rows", PASS_LOSS_ROWS: "loss_rows"}
 CANDIDATES = (8, 16)
 # work-group targets tried per pass kind (0: the built-in default)
-GROUP_CANDIDATES = {PASS_ROW: (0, 256, 512, 2048), PASS_ROW_LOOP: (0, 512, 2048), PASS_COL: (0, 128, 512),
+GROUP_CANDIDATES = {PASS_ROW: (0, 256, 512, 2048), PASS_ROW_LOOP: (0, 1024, 2048), PASS_COL: (0, 128, 512),
                     PASS_COL_LOOP: (0, 128, 512), PASS_VIEW_ROWS: (0, 1024, 4096), PASS_LOSS_ROWS: (0,)}
 GROUPS_SUFFIX = "@groups"
 

@@ -21,10 +21,10 @@ void set_error(const char* fmt, ...) {
 // ---------------------------------------------------------------- tuning table
 static std::atomic<int> g_tune[PASS_KINDS][16];  // 0 = built-in default
 
-static int default_elems(int kind, int64_t n) {
+static int default_elems(int /*kind*/, int64_t n) {
   if (n <= 8) return 8;
   if (n == 16) return 16;
-  return kind == PASS_ROW_LOOP ? 16 : 8;  // PASS_VIEW_ROWS: 8 (five register arrays per thread)
+  return 8;  // every pass kind (PASS_VIEW_ROWS needs it: five register arrays per thread)
 }
 
 static std::atomic<int> g_groups[PASS_KINDS][16];  // 0 = built-in default

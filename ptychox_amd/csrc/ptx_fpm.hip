@@ -443,7 +443,7 @@ struct PlanQuery {
   static ptx_status run(int64_t h, int64_t B, Plan* p) {
     p->S = RowCfg<N, E>::S;
     p->ngroups = (int)((h + p->S - 1) / p->S);
-    int64_t want = (tuned_groups(PASS_ROW_LOOP, N, 1024) + p->ngroups - 1) / p->ngroups;
+    int64_t want = (tuned_groups(PASS_ROW_LOOP, N, kRowLoopGroups) + p->ngroups - 1) / p->ngroups;
     if (want > B) want = B > 0 ? B : 1;
     if (want < 1) want = 1;
     p->chunk = (int)((B + want - 1) / want);

@@ -66,9 +66,12 @@ enum PassKind {
 };
 
 // Elements per thread (8 or 16) the tuning table holds for (kind, n); n a
-// supported power of two.  Defaults come from MI355X measurements (radix-8
+// supported power of two.  Defaults come from MI355X measurements: radix-8
 // passes run ~15 % faster than radix-16 for the plain row / column passes at
-// 256..1024 points, the view-looping row pass prefers radix 16).
+// 256..1024 points, and the view-looping scatter rows (PASS_ROW_LOOP) at
+// radix 8 with kRowLoopGroups work-groups took 58 us against 91 us at radix 16
+// with 1024 (2048^2 object, 512^2 probe, 64 views), the autotuner's choice
+// there and for the 128- and 256-point scatters of mixed workloads.
 int tuned_elems(int kind, int64_t n);
 
 // Work-group target of the passes that walk views (the gathers, the
@@ -76,6 +79,9 @@ int tuned_elems(int kind, int64_t n);
 // views of a launch are spread over, which fixes the chunk of views each
 // work-group walks.  `def` when the table holds no setting for (kind, n).
 int tuned_groups(int kind, int64_t n, int def);
+
+// Built-in work-group target of the PASS_ROW_LOOP scatters.
+constexpr int kRowLoopGroups = 512;
 
 // Consistent view of the tuning tables for the duration of one entry-point
 // call: the first scope on a thread copies both process-wide tables into a

@@ -262,8 +262,8 @@ struct ScatterPlanQuery {
   static ptx_status run(int64_t h, int64_t B, ScatterPlan* p) {
     p->S = RowCfg<N, E>::S;
     p->ngroups = (int)((h + p->S - 1) / p->S);
-    // aim for ~1024 work-groups (tunable); each probe-partial slice covers `chunk` views
-    int64_t want = (tuned_groups(PASS_ROW_LOOP, N, 1024) + p->ngroups - 1) / p->ngroups;
+    // aim for kRowLoopGroups work-groups (tunable); each probe-partial slice covers `chunk` views
+    int64_t want = (tuned_groups(PASS_ROW_LOOP, N, kRowLoopGroups) + p->ngroups - 1) / p->ngroups;
     if (want < 1) want = 1;
     if (want > B) want = B > 0 ? B : 1;
     p->chunk = (int)((B + want - 1) / want);
