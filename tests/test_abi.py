@@ -194,6 +194,29 @@ def test_tuning_table_roundtrip(lib):
     assert autotune.table(autotune.keys_for_scan((512, 512))) == t
 
 
+def test_builtin_schedule_is_radix_8_above_16_points(lib):
+    """With no table entry every pass kind runs 8 elements per thread above
+    16 points, including the view-looping scatter rows (kind 1), whose
+    radix-8 form the autotuner chose on every measured workload."""
+    h = lib.lib()
+    for kind in range(6):
+        for n in (32, 64, 128, 256, 512, 1024, 2048):
+            lib.call("ptx_tune_set", kind, n, 0)  # no entry: the built-in choice
+            assert h.ptx_tune_get(kind, n) == 8, (kind, n)
+        assert h.ptx_tune_get(kind, 16) == 16
+        assert h.ptx_tune_get(kind, 8) == 8
+    # the scatter rows' built-in work-group target is 512: the step's workspace
+    # (its probe-partial slices follow from the target) matches an explicit 512
+    # and differs from 1024
+    args = (2048, 2048, 512, 512, 64, 0)
+    sizes = {}
+    for g in (0, 512, 1024):
+        lib.call("ptx_tune_set_groups", 1, 512, g)
+        sizes[g] = lib.query("ptx_base_step_workspace", *args)
+    lib.call("ptx_tune_set_groups", 1, 512, 0)
+    assert sizes[0] == sizes[512] != sizes[1024]
+
+
 def test_group_table_roundtrip(lib):
     h = lib.lib()
     assert h.ptx_tune_get_groups(1, 512) == 0  # built-in default
