@@ -78,11 +78,63 @@ def check_windows(positions_xy: torch.Tensor, h: int, w: int, H: int, W: int) ->
         return
     lo, hi = torch.aminmax(positions_xy, dim=0)  # one reduction for both columns
     (xmin, ymin), (xmax, ymax) = torch.stack((lo, hi)).tolist()
+    _raise_if_outside(xmin, ymin, xmax, ymax, h, w, H, W)
+
+
+def _raise_if_outside(xmin, ymin, xmax, ymax, h, w, H, W) -> None:
     if xmin < 0 or ymin < 0 or xmax + w > W or ymax + h > H:
         raise ValueError(
             f"scan window out of range: positions x in [{xmin}, {xmax}], y in [{ymin}, {ymax}] with a "
             f"{h}x{w} probe do not fit a {H}x{W} sample."
         )
+
+
+_CHECK_STREAMS: dict[int, torch.cuda.Stream] = {}
+
+
+def begin_window_check(positions_xy: torch.Tensor, h: int, w: int, H: int, W: int):
+    """``check_windows`` in two halves, for a caller that launches the HIP
+    kernels in between (they give an out-of-range window a zero frame and
+    touch no memory outside the sample, so launching them first is safe).
+
+    Positions on the GPU: the min/max reduction runs on a side stream (after
+    the work already queued on the current stream) and its four numbers are
+    copied to page-locked memory; ``end_window_check`` waits for them and
+    raises the same error as ``check_windows``.  The host therefore queues
+    the kernels before it waits, and the device runs them meanwhile.  CPU
+    positions are checked here at once (no synchronisation); an empty batch
+    or a stream capture skips the check.  Returns the handle for
+    ``end_window_check`` (None when nothing is pending)."""
+    if positions_xy.numel() == 0:
+        return None
+    if not positions_xy.is_cuda:
+        check_windows(positions_xy, h, w, H, W)
+        return None
+    if torch.cuda.is_current_stream_capturing():
+        return None
+    dev = positions_xy.device
+    side = _CHECK_STREAMS.get(dev.index)
+    if side is None:
+        side = _CHECK_STREAMS.setdefault(dev.index, torch.cuda.Stream(device=dev))
+    side.wait_stream(torch.cuda.current_stream(dev))
+    host = torch.empty((2, 2), dtype=positions_xy.dtype, pin_memory=True)
+    with torch.cuda.stream(side):
+        lo, hi = torch.aminmax(positions_xy, dim=0)
+        host.copy_(torch.stack((lo, hi)), non_blocking=True)
+        done = torch.cuda.Event()
+        done.record(side)
+    return host, done, (h, w, H, W)
+
+
+def end_window_check(pending) -> None:
+    """Second half of ``begin_window_check``: wait for the four numbers and
+    raise if a scan window leaves the sample."""
+    if pending is None:
+        return
+    host, done, (h, w, H, W) = pending
+    done.synchronize()
+    (xmin, ymin), (xmax, ymax) = host.tolist()
+    _raise_if_outside(xmin, ymin, xmax, ymax, h, w, H, W)
 
 
 def set_scatter_mode(mode: str | None) -> None:

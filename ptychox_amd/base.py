@@ -13,9 +13,11 @@ import torch
 
 from . import _lib
 from ._utils import (
+    begin_window_check,
     can_use_native,
     check_complex_scan_inputs,
     check_windows,
+    end_window_check,
     require_native_frame,
     scatter_flags,
     should_recompute,
@@ -123,16 +125,22 @@ class PtychoXBaseFunction(torch.autograd.Function):
     def forward(ctx, sample, probe, positions_xy, use_cuda_kernels=False):  # type: ignore[override]
         check_complex_scan_inputs(sample, probe, positions_xy)
         h, w = probe.shape
-        # checked before the move to the device: CPU positions cost no synchronisation
-        check_windows(positions_xy, h, w, *sample.shape)
+        native = can_use_native(sample, probe, use_cuda_kernels)
+        # checked before the move to the device: CPU positions cost no
+        # synchronisation; GPU positions on the HIP path are checked beside
+        # the forward kernels (end_window_check below)
+        pending = begin_window_check(positions_xy, h, w, *sample.shape) if native else None
+        if not native:
+            check_windows(positions_xy, h, w, *sample.shape)
         positions_xy = positions_xy.to(device=sample.device, dtype=torch.long).contiguous()
         sample = sample.contiguous()
         probe = probe.contiguous()
         ctx.sample_shape = tuple(sample.shape)
         ctx.recompute = False
-        if can_use_native(sample, probe, use_cuda_kernels):
+        if native:
             require_native_frame(h, w, "probe")
             far_field, amplitude = native_forward(sample, probe, positions_xy)
+            end_window_check(pending)
             # the far field is the one per-pattern tensor kept for the backward
             # (the patches are re-gathered from the sample); recomputed instead
             # when it does not fit (python/ptychoX/base/ops.py:69 saves both)

@@ -16,9 +16,11 @@ import torch
 
 from . import _lib
 from ._utils import (
+    begin_window_check,
     can_use_native,
     check_complex_scan_inputs,
     check_windows,
+    end_window_check,
     should_recompute,
     scatter_flags,
     require_native_frame,
@@ -132,17 +134,23 @@ class PtychoXNearFieldFunction(torch.autograd.Function):
         check_complex_scan_inputs(sample, probe, positions_xy)
         _check_transfer(transfer, probe)
         h, w = probe.shape
-        # checked before the move to the device: CPU positions cost no synchronisation
-        check_windows(positions_xy, h, w, *sample.shape)
+        native = can_use_native(sample, probe, use_cuda_kernels)
+        # checked before the move to the device: CPU positions cost no
+        # synchronisation; GPU positions on the HIP path are checked beside
+        # the forward kernels (end_window_check below)
+        pending = begin_window_check(positions_xy, h, w, *sample.shape) if native else None
+        if not native:
+            check_windows(positions_xy, h, w, *sample.shape)
         positions_xy = positions_xy.to(device=sample.device, dtype=torch.long).contiguous()
         sample = sample.contiguous()
         probe = probe.contiguous()
         transfer = transfer.to(probe.dtype).contiguous()
         ctx.sample_shape = tuple(sample.shape)
         ctx.recompute = False
-        if can_use_native(sample, probe, use_cuda_kernels):
+        if native:
             require_native_frame(h, w, "probe")
             field, amplitude = native_forward(sample, probe, transfer, positions_xy)
+            end_window_check(pending)
             ctx.recompute = should_recompute(field.numel() * 8, sample.device)
             ctx.save_for_backward(sample, probe, transfer, None if ctx.recompute else field, positions_xy)
             ctx.native = True

@@ -185,3 +185,29 @@ def test_base_operator_in_a_captured_graph(gpu):
         torch.cuda.synchronize()
         for a, b in zip(static, eager):
             assert torch.equal(a, b), k
+
+
+@pytest.mark.parametrize("bad", [(-1, 5), (5, -1), (193, 5), (5, 193)])
+def test_gpu_positions_outside_the_sample_raise(gpu, bad):
+    """On the HIP path, positions already on the GPU are checked beside the
+    forward kernels (their reduction runs on a side stream and is waited for
+    after the kernels are queued); the operators still raise the host check's
+    error, and in-range batches give the same amplitudes as CPU positions,
+    which are checked before anything is launched."""
+    from ptychox_amd import angular_spectrum_transfer, ptychoX_base, ptychoX_nearfield
+
+    s, p, pos, _ = _base_inputs(gpu)  # 256^2 sample, 64^2 probe: corners in [0, 192]
+    t = angular_spectrum_transfer(64, 64, 40.0, device=gpu)
+    for op, extra in ((ptychoX_base, ()), (ptychoX_nearfield, (t,))):
+        amp_gpu = op(s, p, *extra, pos, use_cuda_kernels=True)
+        amp_cpu = op(s, p, *extra, pos.cpu(), use_cuda_kernels=True)
+        torch.cuda.synchronize()
+        assert torch.equal(amp_gpu, amp_cpu)
+        outside = pos.clone()
+        outside[3] = torch.tensor(bad, device=gpu)
+        with pytest.raises(ValueError, match="scan window out of range"):
+            op(s, p, *extra, outside, use_cuda_kernels=True)
+        with pytest.raises(ValueError, match="scan window out of range"):
+            op(s, p, *extra, outside.cpu(), use_cuda_kernels=True)
+        # the stream is usable afterwards and the next in-range call is unchanged
+        assert torch.equal(op(s, p, *extra, pos, use_cuda_kernels=True), amp_gpu)
